@@ -1,0 +1,185 @@
+/*
+ * pscbfm.h - C ABI of the MI355X (gfx950) scBFM sweep.
+ *
+ * One handle is one UpdaterGPUScBFM_AB_Type instance of the reference
+ * (src/pscbfm/UpdaterGPUScBFM_AB_Type.h:124-362).  Every entry point returns
+ * PSCBFM_OK or an error code; the message of the last error of a handle is
+ * pscbfm_last_error(handle).  Nothing throws across this boundary and nothing
+ * calls exit(): the reference's C++ exceptions and its exit-on-CUDA-error
+ * (src/pscbfm/cudacommon.hpp:31-46) both become status codes.
+ *
+ * Coordinates are unfolded int32 corners of the 2x2x2 monomer cubes, given
+ * and returned in the caller's monomer order.  Handles are not thread-safe;
+ * separate handles are independent (no process-global device tables).
+ */
+#ifndef PSCBFM_H
+#define PSCBFM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct pscbfm_updater pscbfm_updater;
+
+enum pscbfm_status {
+    PSCBFM_OK = 0,
+    PSCBFM_INVALID_ARGUMENT = 1, /* std::invalid_argument in the reference */
+    PSCBFM_RUNTIME_ERROR = 2,    /* std::runtime_error in the reference */
+    PSCBFM_OUT_OF_RANGE = 3,     /* std::out_of_range (vector::at) in the reference */
+    PSCBFM_DEVICE_ERROR = 4,     /* HIP runtime failure (the reference exits) */
+    PSCBFM_NULL_HANDLE = 5,
+    PSCBFM_INTERNAL_ERROR = 6
+};
+
+enum pscbfm_lattice_kind {
+    PSCBFM_LATTICE_U8 = 0,  /* one byte per cell */
+    PSCBFM_LATTICE_BIT = 1  /* one bit per cell (default) */
+};
+
+/* Constructor UpdaterGPUScBFM_AB_Type() (UpdaterGPUScBFM_AB_Type.h:316). */
+int pscbfm_create(pscbfm_updater** out);
+/* Destructor ~UpdaterGPUScBFM_AB_Type() (UpdaterGPUScBFM_AB_Type.h:317). */
+void pscbfm_destroy(pscbfm_updater* h);
+/* Message of the last failed call on h (empty if none); valid until the next call. */
+const char* pscbfm_last_error(const pscbfm_updater* h);
+/* Message of the last failed pscbfm_create in this thread. */
+const char* pscbfm_create_error(void);
+
+/* destruct() (UpdaterGPUScBFM_AB_Type.h:318). */
+int pscbfm_destruct(pscbfm_updater* h);
+/* setGpu(int) (UpdaterGPUScBFM_AB_Type.h:340). */
+int pscbfm_set_gpu(pscbfm_updater* h, int gpu);
+/* copyBondSet(dx,dy,dz,forbidden) (UpdaterGPUScBFM_AB_Type.h:341). */
+int pscbfm_copy_bond_set(pscbfm_updater* h, int dx, int dy, int dz, int forbidden);
+/* setNrOfAllMonomers(n) (UpdaterGPUScBFM_AB_Type.h:342). */
+int pscbfm_set_nr_of_all_monomers(pscbfm_updater* h, uint32_t n);
+/* setAttribute(i, a) (UpdaterGPUScBFM_AB_Type.h:343). */
+int pscbfm_set_attribute(pscbfm_updater* h, uint32_t i, int32_t attribute);
+/* setMonomerCoordinates(i,x,y,z) (UpdaterGPUScBFM_AB_Type.h:344). */
+int pscbfm_set_monomer_coordinates(pscbfm_updater* h, uint32_t i, int32_t x, int32_t y, int32_t z);
+/* Bulk form of setMonomerCoordinates: xyz holds 3*n values. */
+int pscbfm_set_monomer_coordinates_bulk(pscbfm_updater* h, const int32_t* xyz, uint32_t n);
+/* setConnectivity(i, j), directed (UpdaterGPUScBFM_AB_Type.h:345). */
+int pscbfm_set_connectivity(pscbfm_updater* h, uint32_t i, uint32_t j);
+/* Bulk form of setConnectivity: pairs holds 2*n_pairs ids, each pair directed. */
+int pscbfm_set_connectivity_bulk(pscbfm_updater* h, const uint32_t* pairs, uint64_t n_pairs);
+/* setLatticeSize(x,y,z) (UpdaterGPUScBFM_AB_Type.h:346). */
+int pscbfm_set_lattice_size(pscbfm_updater* h, uint32_t x, uint32_t y, uint32_t z);
+/* setPeriodicity(x,y,z) (UpdaterGPUScBFM_AB_Type.h:358). */
+int pscbfm_set_periodicity(pscbfm_updater* h, int x, int y, int z);
+/* populateLattice() (UpdaterGPUScBFM_AB_Type.h:351). */
+int pscbfm_populate_lattice(pscbfm_updater* h);
+/* initialize() (UpdaterGPUScBFM_AB_Type.h:335). */
+int pscbfm_initialize(pscbfm_updater* h);
+/* execute() (UpdaterGPUScBFM_AB_Type.h:336): a no-op that succeeds. */
+int pscbfm_execute(pscbfm_updater* h);
+/* runSimulationOnGPU(nMCS) (UpdaterGPUScBFM_AB_Type.h:352; CRITICAL.cu:414-698).
+ * Synchronous.  accepted_out (may be NULL) receives the accepted moves of this call. */
+int pscbfm_run(pscbfm_updater* h, int32_t n_mcs, uint64_t* accepted_out);
+/* getMonomerPositionInX/Y/Z(i) (UpdaterGPUScBFM_AB_Type.h:354-356). */
+int pscbfm_get_monomer_position(const pscbfm_updater* h, uint32_t i, int32_t* x, int32_t* y, int32_t* z);
+/* Bulk form of the getters: xyz receives 3*n values in monomer order. */
+int pscbfm_get_positions_bulk(const pscbfm_updater* h, int32_t* xyz, uint32_t n);
+/* checkSystem() (UpdaterGPUScBFM_AB_Type.h:313, private there): excluded
+ * volume and bond-set invariants of the current coordinates. */
+int pscbfm_check_system(const pscbfm_updater* h);
+/* cleanup() (UpdaterGPUScBFM_AB_Type.h:337). */
+int pscbfm_cleanup(pscbfm_updater* h);
+
+/* ---- Extensions (no single reference counterpart) -------------------------- */
+
+/* Key of the counter-based substep draw; replaces the seeding of LeMonADE's
+ * R250 stream that drives CRITICAL.cu:509-510.  Before initialize. */
+int pscbfm_set_seed(pscbfm_updater* h, uint64_t seed);
+/* Occupancy lattice storage (pscbfm_lattice_kind).  Before initialize.
+ * Replaces the USE_BIT_PACKING_LATTICE switch (UpdaterGPUScBFM_AB_Type.cu:13). */
+int pscbfm_set_lattice_kind(pscbfm_updater* h, int kind);
+/* Uniform color rebalancing on/off (UpdaterGPUScBFM_AB_Type.cu:550, default on). */
+int pscbfm_set_uniform_coloring(pscbfm_updater* h, int on);
+/* Run checkSystem inside initialize (default on). */
+int pscbfm_set_check_on_initialize(pscbfm_updater* h, int on);
+/* Replace the automatic coloring with a proper coloring given per monomer. */
+int pscbfm_set_colors(pscbfm_updater* h, const uint8_t* colors, uint32_t n);
+/* Threads per workgroup (multiple of 64, 0 = 256) and workgroup caps of the
+ * two kernels (0 = default). */
+int pscbfm_set_launch_shape(pscbfm_updater* h, uint32_t block, uint32_t check_cap, uint32_t perform_cap);
+/* Substeps per captured graph (0 = automatic). */
+int pscbfm_set_graph_chunk(pscbfm_updater* h, uint32_t substeps);
+
+/* Statistics of the last run. */
+typedef struct pscbfm_run_stats {
+    double gpu_ms;            /* hipEvent time of all launched substeps */
+    uint64_t substeps;
+    uint64_t attempted;       /* sum of species sizes over the substeps */
+    uint64_t accepted;
+    uint64_t bond_terms;      /* sum over substeps of the drawn species' bond count */
+    double check_ms;          /* pass-1 kernel time (profiled runs only) */
+    double perform_ms;        /* pass-2 kernel time (profiled runs only) */
+    uint64_t check_launches;
+    uint64_t perform_launches;
+} pscbfm_run_stats;
+
+/* Run with options: log (may be NULL) receives n_mcs*n_colors per-substep
+ * accepted counts; profile!=0 times both kernels of every substep with HIP
+ * events (eager launches); use_graph selects hipGraph replay. */
+int pscbfm_run_ex(pscbfm_updater* h, int32_t n_mcs, uint32_t* log, int profile, int use_graph,
+                  pscbfm_run_stats* stats);
+int pscbfm_last_run_stats(const pscbfm_updater* h, pscbfm_run_stats* stats);
+
+/* Layout introspection: number of species, per-monomer species, species sizes. */
+int pscbfm_get_n_colors(const pscbfm_updater* h, uint32_t* n_colors);
+int pscbfm_get_colors(const pscbfm_updater* h, uint8_t* colors, uint32_t n);
+int pscbfm_get_color_sizes(const pscbfm_updater* h, uint32_t* sizes, uint32_t n_colors);
+/* Substep counter (substeps done since initialize) and accepted moves since initialize. */
+int pscbfm_get_counters(const pscbfm_updater* h, uint64_t* substeps, uint64_t* accepted);
+/* Device bytes held by the handle. */
+int pscbfm_get_device_bytes(const pscbfm_updater* h, uint64_t* bytes);
+/* Device occupancy lattice as one byte per cell in Z-curve order (cells bytes). */
+int pscbfm_download_lattice(const pscbfm_updater* h, uint8_t* cells, uint64_t n_cells);
+/* Number of set bits in both proposal lattices (0 after every complete substep pair
+ * once the next pass 1 has cleared the older buffer). */
+int pscbfm_count_proposal_bits(const pscbfm_updater* h, uint64_t* bits_even, uint64_t* bits_odd);
+
+/* Host helpers shared with the kernels (no device work). */
+/* Z-curve code of wrapped coordinates for a box (bit layout of the device lattice). */
+int pscbfm_zcurve_encode(uint32_t lx, uint32_t ly, uint32_t lz, uint32_t x, uint32_t y, uint32_t z,
+                         uint32_t* code);
+/* Move hash of CRITICAL.cu:2-11. */
+uint32_t pscbfm_move_hash(uint32_t a);
+/* Species and kernel seed of substep t (replaces CRITICAL.cu:509-510). */
+void pscbfm_substep_draw(uint64_t seed, uint64_t t, uint32_t n_colors, uint32_t* species, uint32_t* kernel_seed);
+/* Graph coloring of graphColoring.tpp:77-293 over a CSR adjacency. */
+int pscbfm_color_graph(uint32_t n, const uint32_t* start, const uint32_t* ids, int uniform, uint8_t* colors);
+/* The 108 scBFM bond vectors (3 ints each) in a fixed order. */
+int pscbfm_standard_bond_vectors(int32_t* xyz108);
+
+/* ---- Synthetic systems (host only, seeded, deterministic) ------------------ */
+
+/* Linear chains grown as self-avoiding walks with bonds drawn uniformly from
+ * the 108-set; chain c is monomers [c*len, (c+1)*len), bonded consecutively.
+ * xyz receives 3*n_chains*len unfolded corners.  PSCBFM_RUNTIME_ERROR if the
+ * growth needed more than max_restarts chain restarts (box too dense). */
+int pscbfm_gen_linear_chains(uint32_t lx, uint32_t ly, uint32_t lz, uint32_t n_chains, uint32_t len,
+                             uint64_t seed, uint32_t max_restarts, int32_t* xyz);
+/* Linear chains on an ordered serpentine packing (corners on (2i,2j,3k));
+ * supports occupied-site fractions up to 2*floor(lz/3)/lz. */
+int pscbfm_gen_linear_chains_ordered(uint32_t lx, uint32_t ly, uint32_t lz, uint32_t n_chains, uint32_t len,
+                                     int32_t* xyz);
+/* Star network: n_stars stars of `arms` arms of arm_length monomers (center
+ * first), arm ends crosslinked to other stars' arm interiors.  xyz receives
+ * 3*n_stars*(1+arms*arm_length) corners; bonds receives undirected pairs
+ * (2 ids each, capacity in pairs: n_stars*arms*(arm_length+1)). */
+int pscbfm_gen_star_network(uint32_t lx, uint32_t ly, uint32_t lz, uint32_t n_stars, uint32_t arms,
+                            uint32_t arm_length, uint64_t seed, int32_t* xyz, uint32_t* bonds,
+                            uint64_t bond_capacity, uint64_t* n_bonds);
+
+/* Library version string. */
+const char* pscbfm_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PSCBFM_H */

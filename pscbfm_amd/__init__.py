@@ -1,0 +1,393 @@
+"""MI355X (gfx950) scBFM sweep: Python mirror of the reference's updater API.
+
+`UpdaterGPUScBFM_AB_Type` keeps the method names, argument meaning and error
+behaviour of src/pscbfm/UpdaterGPUScBFM_AB_Type.h:316-361, and calls the
+hand-written HIP kernels through the C ABI of include/pscbfm.h (ctypes over
+pscbfm_amd/libpscbfm.so).  There is no fallback: without the built library
+every call raises.
+
+Error mapping (reference exception -> Python):
+    std::invalid_argument -> InvalidArgument (a ValueError)
+    std::runtime_error    -> RuntimeError
+    std::out_of_range     -> IndexError
+    CUDA error + exit()   -> DeviceError (a RuntimeError)
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+from typing import Optional, Sequence
+
+import numpy as np
+
+__all__ = [
+    "UpdaterGPUScBFM_AB_Type", "InvalidArgument", "DeviceError", "RunStats", "lib", "library_path",
+    "LATTICE_U8", "LATTICE_BIT", "standard_bond_vectors", "move_hash", "substep_draw",
+    "zcurve_encode", "color_graph", "gen_linear_chains", "gen_linear_chains_ordered",
+    "gen_star_network", "chain_bonds",
+]
+
+LATTICE_U8 = 0
+LATTICE_BIT = 1
+
+_HERE = Path(__file__).resolve().parent
+library_path = _HERE / "libpscbfm.so"
+
+
+class InvalidArgument(ValueError):
+    """std::invalid_argument of the reference."""
+
+
+class DeviceError(RuntimeError):
+    """HIP runtime failure (the reference exits the process instead)."""
+
+
+class RunStats(C.Structure):
+    _fields_ = [
+        ("gpu_ms", C.c_double), ("substeps", C.c_uint64), ("attempted", C.c_uint64),
+        ("accepted", C.c_uint64), ("bond_terms", C.c_uint64), ("check_ms", C.c_double),
+        ("perform_ms", C.c_double), ("check_launches", C.c_uint64), ("perform_launches", C.c_uint64),
+    ]
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+_LIB: Optional[C.CDLL] = None
+
+_u32p = C.POINTER(C.c_uint32)
+_i32p = C.POINTER(C.c_int32)
+_u64p = C.POINTER(C.c_uint64)
+_u8p = C.POINTER(C.c_uint8)
+_H = C.c_void_p
+
+_SIGNATURES = {
+    "pscbfm_create": (C.c_int, [C.POINTER(_H)]),
+    "pscbfm_destroy": (None, [_H]),
+    "pscbfm_last_error": (C.c_char_p, [_H]),
+    "pscbfm_create_error": (C.c_char_p, []),
+    "pscbfm_destruct": (C.c_int, [_H]),
+    "pscbfm_set_gpu": (C.c_int, [_H, C.c_int]),
+    "pscbfm_copy_bond_set": (C.c_int, [_H, C.c_int, C.c_int, C.c_int, C.c_int]),
+    "pscbfm_set_nr_of_all_monomers": (C.c_int, [_H, C.c_uint32]),
+    "pscbfm_set_attribute": (C.c_int, [_H, C.c_uint32, C.c_int32]),
+    "pscbfm_set_monomer_coordinates": (C.c_int, [_H, C.c_uint32, C.c_int32, C.c_int32, C.c_int32]),
+    "pscbfm_set_monomer_coordinates_bulk": (C.c_int, [_H, _i32p, C.c_uint32]),
+    "pscbfm_set_connectivity": (C.c_int, [_H, C.c_uint32, C.c_uint32]),
+    "pscbfm_set_connectivity_bulk": (C.c_int, [_H, _u32p, C.c_uint64]),
+    "pscbfm_set_lattice_size": (C.c_int, [_H, C.c_uint32, C.c_uint32, C.c_uint32]),
+    "pscbfm_set_periodicity": (C.c_int, [_H, C.c_int, C.c_int, C.c_int]),
+    "pscbfm_populate_lattice": (C.c_int, [_H]),
+    "pscbfm_initialize": (C.c_int, [_H]),
+    "pscbfm_execute": (C.c_int, [_H]),
+    "pscbfm_run": (C.c_int, [_H, C.c_int32, _u64p]),
+    "pscbfm_get_monomer_position": (C.c_int, [_H, C.c_uint32, _i32p, _i32p, _i32p]),
+    "pscbfm_get_positions_bulk": (C.c_int, [_H, _i32p, C.c_uint32]),
+    "pscbfm_check_system": (C.c_int, [_H]),
+    "pscbfm_cleanup": (C.c_int, [_H]),
+    "pscbfm_set_seed": (C.c_int, [_H, C.c_uint64]),
+    "pscbfm_set_lattice_kind": (C.c_int, [_H, C.c_int]),
+    "pscbfm_set_uniform_coloring": (C.c_int, [_H, C.c_int]),
+    "pscbfm_set_check_on_initialize": (C.c_int, [_H, C.c_int]),
+    "pscbfm_set_colors": (C.c_int, [_H, _u8p, C.c_uint32]),
+    "pscbfm_set_launch_shape": (C.c_int, [_H, C.c_uint32, C.c_uint32, C.c_uint32]),
+    "pscbfm_set_graph_chunk": (C.c_int, [_H, C.c_uint32]),
+    "pscbfm_run_ex": (C.c_int, [_H, C.c_int32, _u32p, C.c_int, C.c_int, C.POINTER(RunStats)]),
+    "pscbfm_last_run_stats": (C.c_int, [_H, C.POINTER(RunStats)]),
+    "pscbfm_get_n_colors": (C.c_int, [_H, _u32p]),
+    "pscbfm_get_colors": (C.c_int, [_H, _u8p, C.c_uint32]),
+    "pscbfm_get_color_sizes": (C.c_int, [_H, _u32p, C.c_uint32]),
+    "pscbfm_get_counters": (C.c_int, [_H, _u64p, _u64p]),
+    "pscbfm_get_device_bytes": (C.c_int, [_H, _u64p]),
+    "pscbfm_download_lattice": (C.c_int, [_H, _u8p, C.c_uint64]),
+    "pscbfm_count_proposal_bits": (C.c_int, [_H, _u64p, _u64p]),
+    "pscbfm_zcurve_encode": (C.c_int, [C.c_uint32] * 6 + [_u32p]),
+    "pscbfm_move_hash": (C.c_uint32, [C.c_uint32]),
+    "pscbfm_substep_draw": (None, [C.c_uint64, C.c_uint64, C.c_uint32, _u32p, _u32p]),
+    "pscbfm_color_graph": (C.c_int, [C.c_uint32, _u32p, _u32p, C.c_int, _u8p]),
+    "pscbfm_standard_bond_vectors": (C.c_int, [_i32p]),
+    "pscbfm_gen_linear_chains": (C.c_int, [C.c_uint32] * 5 + [C.c_uint64, C.c_uint32, _i32p]),
+    "pscbfm_gen_linear_chains_ordered": (C.c_int, [C.c_uint32] * 5 + [_i32p]),
+    "pscbfm_gen_star_network": (C.c_int, [C.c_uint32] * 6 + [C.c_uint64, _i32p, _u32p, C.c_uint64, _u64p]),
+    "pscbfm_version": (C.c_char_p, []),
+}
+
+
+def lib() -> C.CDLL:
+    """Loads the in-tree HIP library (raises if it has not been built)."""
+    global _LIB
+    if _LIB is None:
+        if not library_path.exists():
+            raise RuntimeError(f"{library_path} is missing; build it with `python -m pscbfm_amd.build` "
+                               "(the sweep has no CPU fallback)")
+        handle = C.CDLL(str(library_path))
+        for name, (res, args) in _SIGNATURES.items():
+            fn = getattr(handle, name)
+            fn.restype = res
+            fn.argtypes = args
+        _LIB = handle
+    return _LIB
+
+
+def _ptr(arr: np.ndarray, ctype):
+    return arr.ctypes.data_as(C.POINTER(ctype))
+
+
+def _raise(code: int, message: str) -> None:
+    if code == 0:
+        return
+    if code == 1:
+        raise InvalidArgument(message)
+    if code == 2:
+        raise RuntimeError(message)
+    if code == 3:
+        raise IndexError(message)
+    if code == 4:
+        raise DeviceError(message)
+    raise RuntimeError(f"pscbfm error {code}: {message}")
+
+
+class UpdaterGPUScBFM_AB_Type:
+    """Drop-in mirror of the reference class (UpdaterGPUScBFM_AB_Type.h:124-362)."""
+
+    def __init__(self) -> None:
+        self._lib = lib()
+        h = C.c_void_p()
+        code = self._lib.pscbfm_create(C.byref(h))
+        _raise(code, (self._lib.pscbfm_create_error() or b"").decode())
+        self._h = h
+
+    def __del__(self) -> None:
+        h = getattr(self, "_h", None)
+        if h:
+            self._lib.pscbfm_destroy(h)
+            self._h = None
+
+    def _call(self, name: str, *args) -> None:
+        code = getattr(self._lib, name)(self._h, *args)
+        if code:
+            _raise(code, (self._lib.pscbfm_last_error(self._h) or b"").decode())
+
+    # -- reference surface ----------------------------------------------------
+    def destruct(self) -> None:
+        self._call("pscbfm_destruct")
+
+    def setGpu(self, iGpuToUse: int) -> None:
+        self._call("pscbfm_set_gpu", int(iGpuToUse))
+
+    def copyBondSet(self, dx: int, dy: int, dz: int, bondForbidden: bool) -> None:
+        self._call("pscbfm_copy_bond_set", int(dx), int(dy), int(dz), int(bool(bondForbidden)))
+
+    def setNrOfAllMonomers(self, nAllMonomers: int) -> None:
+        self._call("pscbfm_set_nr_of_all_monomers", int(nAllMonomers))
+
+    def setAttribute(self, i: int, attribute: int) -> None:
+        self._call("pscbfm_set_attribute", int(i), int(attribute))
+
+    def setMonomerCoordinates(self, i: int, x: int, y: int, z: int) -> None:
+        self._call("pscbfm_set_monomer_coordinates", int(i), int(x), int(y), int(z))
+
+    def setConnectivity(self, monoidx1: int, monoidx2: int) -> None:
+        self._call("pscbfm_set_connectivity", int(monoidx1), int(monoidx2))
+
+    def setLatticeSize(self, boxX: int, boxY: int, boxZ: int) -> None:
+        self._call("pscbfm_set_lattice_size", int(boxX), int(boxY), int(boxZ))
+
+    def setPeriodicity(self, isPeriodicX: bool, isPeriodicY: bool, isPeriodicZ: bool) -> None:
+        self._call("pscbfm_set_periodicity", int(bool(isPeriodicX)), int(bool(isPeriodicY)),
+                   int(bool(isPeriodicZ)))
+
+    def populateLattice(self) -> None:
+        self._call("pscbfm_populate_lattice")
+
+    def initialize(self) -> None:
+        self._call("pscbfm_initialize")
+
+    def execute(self) -> bool:
+        self._call("pscbfm_execute")
+        return True
+
+    def runSimulationOnGPU(self, nrMCS_per_Call: int) -> None:
+        self._call("pscbfm_run", int(nrMCS_per_Call), None)
+
+    def getMonomerPositionInX(self, i: int) -> int:
+        return self._position(i)[0]
+
+    def getMonomerPositionInY(self, i: int) -> int:
+        return self._position(i)[1]
+
+    def getMonomerPositionInZ(self, i: int) -> int:
+        return self._position(i)[2]
+
+    def checkSystem(self) -> None:
+        self._call("pscbfm_check_system")
+
+    def cleanup(self) -> None:
+        self._call("pscbfm_cleanup")
+
+    def _position(self, i: int):
+        x, y, z = C.c_int32(), C.c_int32(), C.c_int32()
+        self._call("pscbfm_get_monomer_position", int(i), C.byref(x), C.byref(y), C.byref(z))
+        return x.value, y.value, z.value
+
+    # -- extensions -----------------------------------------------------------
+    def setSeed(self, seed: int) -> None:
+        self._call("pscbfm_set_seed", int(seed) & 0xFFFFFFFFFFFFFFFF)
+
+    def setLatticeKind(self, kind: int) -> None:
+        self._call("pscbfm_set_lattice_kind", int(kind))
+
+    def setUniformColoring(self, on: bool) -> None:
+        self._call("pscbfm_set_uniform_coloring", int(bool(on)))
+
+    def setCheckOnInitialize(self, on: bool) -> None:
+        self._call("pscbfm_set_check_on_initialize", int(bool(on)))
+
+    def setColors(self, colors: Sequence[int]) -> None:
+        arr = np.ascontiguousarray(colors, dtype=np.uint8)
+        self._call("pscbfm_set_colors", _ptr(arr, C.c_uint8), int(arr.size))
+
+    def setLaunchShape(self, block: int = 0, check_cap: int = 0, perform_cap: int = 0) -> None:
+        self._call("pscbfm_set_launch_shape", int(block), int(check_cap), int(perform_cap))
+
+    def setGraphChunk(self, substeps: int) -> None:
+        self._call("pscbfm_set_graph_chunk", int(substeps))
+
+    def setMonomerCoordinatesBulk(self, xyz: np.ndarray) -> None:
+        arr = np.ascontiguousarray(xyz, dtype=np.int32).reshape(-1, 3)
+        self._call("pscbfm_set_monomer_coordinates_bulk", _ptr(arr, C.c_int32), int(arr.shape[0]))
+
+    def setConnectivityBulk(self, pairs: np.ndarray) -> None:
+        arr = np.ascontiguousarray(pairs, dtype=np.uint32).reshape(-1, 2)
+        self._call("pscbfm_set_connectivity_bulk", _ptr(arr, C.c_uint32), int(arr.shape[0]))
+
+    def getPositions(self, n: int) -> np.ndarray:
+        out = np.empty((n, 3), dtype=np.int32)
+        self._call("pscbfm_get_positions_bulk", _ptr(out, C.c_int32), int(n))
+        return out
+
+    def run(self, n_mcs: int, log: bool = False, profile: bool = False, use_graph: bool = True):
+        """Runs n_mcs sweeps; returns (RunStats, per-substep accepted log or None)."""
+        stats = RunStats()
+        n_colors = self.nColors() if log else 0
+        buf = np.zeros(int(n_mcs) * n_colors, dtype=np.uint32) if log else None
+        self._call("pscbfm_run_ex", int(n_mcs), _ptr(buf, C.c_uint32) if log else None,
+                   int(bool(profile)), int(bool(use_graph)), C.byref(stats))
+        return stats, buf
+
+    def lastRunStats(self) -> RunStats:
+        s = RunStats()
+        self._call("pscbfm_last_run_stats", C.byref(s))
+        return s
+
+    def nColors(self) -> int:
+        v = C.c_uint32()
+        self._call("pscbfm_get_n_colors", C.byref(v))
+        return v.value
+
+    def colors(self, n: int) -> np.ndarray:
+        out = np.empty(n, dtype=np.uint8)
+        self._call("pscbfm_get_colors", _ptr(out, C.c_uint8), int(n))
+        return out
+
+    def colorSizes(self) -> np.ndarray:
+        k = self.nColors()
+        out = np.empty(k, dtype=np.uint32)
+        self._call("pscbfm_get_color_sizes", _ptr(out, C.c_uint32), int(k))
+        return out
+
+    def counters(self):
+        s, a = C.c_uint64(), C.c_uint64()
+        self._call("pscbfm_get_counters", C.byref(s), C.byref(a))
+        return s.value, a.value
+
+    def deviceBytes(self) -> int:
+        v = C.c_uint64()
+        self._call("pscbfm_get_device_bytes", C.byref(v))
+        return v.value
+
+    def downloadLattice(self, n_cells: int) -> np.ndarray:
+        out = np.empty(n_cells, dtype=np.uint8)
+        self._call("pscbfm_download_lattice", _ptr(out, C.c_uint8), int(n_cells))
+        return out
+
+    def proposalBits(self):
+        a, b = C.c_uint64(), C.c_uint64()
+        self._call("pscbfm_count_proposal_bits", C.byref(a), C.byref(b))
+        return a.value, b.value
+
+
+# -- host helpers -----------------------------------------------------------------
+def standard_bond_vectors() -> np.ndarray:
+    out = np.empty((108, 3), dtype=np.int32)
+    _raise(lib().pscbfm_standard_bond_vectors(_ptr(out, C.c_int32)), "bond vectors")
+    return out
+
+
+def move_hash(a: int) -> int:
+    return int(lib().pscbfm_move_hash(int(a) & 0xFFFFFFFF))
+
+
+def substep_draw(seed: int, t: int, n_colors: int):
+    s, k = C.c_uint32(), C.c_uint32()
+    lib().pscbfm_substep_draw(int(seed) & 0xFFFFFFFFFFFFFFFF, int(t), int(n_colors), C.byref(s), C.byref(k))
+    return s.value, k.value
+
+
+def zcurve_encode(box, x: int, y: int, z: int) -> int:
+    out = C.c_uint32()
+    _raise(lib().pscbfm_zcurve_encode(int(box[0]), int(box[1]), int(box[2]), int(x), int(y), int(z),
+                                      C.byref(out)), "zcurve")
+    return out.value
+
+
+def color_graph(n: int, start: np.ndarray, ids: np.ndarray, uniform: bool = True) -> np.ndarray:
+    s = np.ascontiguousarray(start, dtype=np.uint32)
+    i = np.ascontiguousarray(ids, dtype=np.uint32)
+    if i.size == 0:
+        i = np.zeros(1, dtype=np.uint32)
+    out = np.empty(max(n, 1), dtype=np.uint8)
+    _raise(lib().pscbfm_color_graph(int(n), _ptr(s, C.c_uint32), _ptr(i, C.c_uint32), int(bool(uniform)),
+                                    _ptr(out, C.c_uint8)), "graph coloring")
+    return out[:n]
+
+
+def chain_bonds(n_chains: int, length: int) -> np.ndarray:
+    """Undirected bond pairs (i, i+1) inside each of n_chains consecutive chains."""
+    if length < 2 or n_chains == 0:
+        return np.zeros((0, 2), dtype=np.uint32)
+    base = (np.arange(n_chains, dtype=np.uint64) * length)[:, None]
+    k = np.arange(length - 1, dtype=np.uint64)[None, :]
+    a = (base + k).reshape(-1)
+    return np.stack([a, a + 1], axis=1).astype(np.uint32)
+
+
+def gen_linear_chains(box, n_chains: int, length: int, seed: int, max_restarts: int = 10000) -> np.ndarray:
+    """Chains grown as self-avoiding walks (raises RuntimeError if the box jams)."""
+    out = np.empty((n_chains * length, 3), dtype=np.int32)
+    code = lib().pscbfm_gen_linear_chains(int(box[0]), int(box[1]), int(box[2]), int(n_chains), int(length),
+                                          int(seed) & 0xFFFFFFFFFFFFFFFF, int(max_restarts),
+                                          _ptr(out, C.c_int32))
+    _raise(code, "self-avoiding growth failed (box too dense for the requested chains)")
+    return out
+
+
+def gen_linear_chains_ordered(box, n_chains: int, length: int) -> np.ndarray:
+    out = np.empty((n_chains * length, 3), dtype=np.int32)
+    code = lib().pscbfm_gen_linear_chains_ordered(int(box[0]), int(box[1]), int(box[2]), int(n_chains),
+                                                  int(length), _ptr(out, C.c_int32))
+    _raise(code, "ordered placement does not fit the requested chains")
+    return out
+
+
+def gen_star_network(box, n_stars: int, arms: int, arm_length: int, seed: int):
+    per = 1 + arms * arm_length
+    xyz = np.empty((n_stars * per, 3), dtype=np.int32)
+    cap = n_stars * arms * (arm_length + 1)
+    bonds = np.empty((max(cap, 1), 2), dtype=np.uint32)
+    nb = C.c_uint64()
+    code = lib().pscbfm_gen_star_network(int(box[0]), int(box[1]), int(box[2]), int(n_stars), int(arms),
+                                         int(arm_length), int(seed) & 0xFFFFFFFFFFFFFFFF, _ptr(xyz, C.c_int32),
+                                         _ptr(bonds, C.c_uint32), int(cap), C.byref(nb))
+    _raise(code, "star network growth failed")
+    return xyz, bonds[: nb.value].copy()

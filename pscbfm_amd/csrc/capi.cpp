@@ -1,0 +1,390 @@
+// extern "C" boundary of the scBFM sweep (include/pscbfm.h).  Exceptions of
+// the engine become status codes plus a per-handle message.
+#include "../../include/pscbfm.h"
+
+#include <cstring>
+#include <new>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "coloring.h"
+#include "common.h"
+#include "generators.h"
+#include "updater.h"
+
+struct pscbfm_updater {
+    pscbfm::Updater engine;
+    std::string error;
+};
+
+namespace {
+
+thread_local std::string g_createError;
+
+template <class F>
+int guarded(pscbfm_updater* h, F&& f) {
+    if (!h) return PSCBFM_NULL_HANDLE;
+    h->error.clear();
+    try {
+        f(h->engine);
+        return PSCBFM_OK;
+    } catch (const std::invalid_argument& e) {
+        h->error = e.what();
+        return PSCBFM_INVALID_ARGUMENT;
+    } catch (const std::out_of_range& e) {
+        h->error = e.what();
+        return PSCBFM_OUT_OF_RANGE;
+    } catch (const std::runtime_error& e) {
+        h->error = e.what();
+        return std::strncmp(e.what(), "HIP error", 9) == 0 ? PSCBFM_DEVICE_ERROR : PSCBFM_RUNTIME_ERROR;
+    } catch (const std::bad_alloc&) {
+        h->error = "host allocation failed";
+        return PSCBFM_RUNTIME_ERROR;
+    } catch (const std::exception& e) {
+        h->error = e.what();
+        return PSCBFM_INTERNAL_ERROR;
+    } catch (...) {
+        h->error = "unknown exception";
+        return PSCBFM_INTERNAL_ERROR;
+    }
+}
+
+template <class F>
+int guarded_const(const pscbfm_updater* h, F&& f) {
+    return guarded(const_cast<pscbfm_updater*>(h), [&](pscbfm::Updater& u) { f(static_cast<const pscbfm::Updater&>(u)); });
+}
+
+void fill_stats(const pscbfm::RunStats& r, pscbfm_run_stats* s) {
+    s->gpu_ms = r.gpuMilliseconds;
+    s->substeps = r.substeps;
+    s->attempted = r.attempted;
+    s->accepted = r.accepted;
+    s->bond_terms = r.bondTerms;
+    s->check_ms = r.checkMilliseconds;
+    s->perform_ms = r.performMilliseconds;
+    s->check_launches = r.checkLaunches;
+    s->perform_launches = r.performLaunches;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pscbfm_create(pscbfm_updater** out) {
+    if (!out) return PSCBFM_INVALID_ARGUMENT;
+    *out = nullptr;
+    try {
+        *out = new pscbfm_updater();
+        return PSCBFM_OK;
+    } catch (const std::exception& e) {
+        g_createError = e.what();
+        return PSCBFM_RUNTIME_ERROR;
+    }
+}
+
+void pscbfm_destroy(pscbfm_updater* h) { delete h; }
+
+const char* pscbfm_last_error(const pscbfm_updater* h) { return h ? h->error.c_str() : "null handle"; }
+
+const char* pscbfm_create_error(void) { return g_createError.c_str(); }
+
+int pscbfm_destruct(pscbfm_updater* h) {
+    return guarded(h, [](pscbfm::Updater& u) { u.destruct(); });
+}
+
+int pscbfm_set_gpu(pscbfm_updater* h, int gpu) {
+    return guarded(h, [&](pscbfm::Updater& u) { u.setGpu(gpu); });
+}
+
+int pscbfm_copy_bond_set(pscbfm_updater* h, int dx, int dy, int dz, int forbidden) {
+    return guarded(h, [&](pscbfm::Updater& u) { u.copyBondSet(dx, dy, dz, forbidden != 0); });
+}
+
+int pscbfm_set_nr_of_all_monomers(pscbfm_updater* h, uint32_t n) {
+    return guarded(h, [&](pscbfm::Updater& u) { u.setNrOfAllMonomers(n); });
+}
+
+int pscbfm_set_attribute(pscbfm_updater* h, uint32_t i, int32_t attribute) {
+    return guarded(h, [&](pscbfm::Updater& u) { u.setAttribute(i, attribute); });
+}
+
+int pscbfm_set_monomer_coordinates(pscbfm_updater* h, uint32_t i, int32_t x, int32_t y, int32_t z) {
+    return guarded(h, [&](pscbfm::Updater& u) { u.setMonomerCoordinates(i, x, y, z); });
+}
+
+int pscbfm_set_monomer_coordinates_bulk(pscbfm_updater* h, const int32_t* xyz, uint32_t n) {
+    return guarded(h, [&](pscbfm::Updater& u) {
+        if (!xyz && n) throw std::invalid_argument("null coordinate array");
+        u.setMonomerCoordinatesBulk(xyz, n);
+    });
+}
+
+int pscbfm_set_connectivity(pscbfm_updater* h, uint32_t i, uint32_t j) {
+    return guarded(h, [&](pscbfm::Updater& u) { u.setConnectivity(i, j); });
+}
+
+int pscbfm_set_connectivity_bulk(pscbfm_updater* h, const uint32_t* pairs, uint64_t n_pairs) {
+    return guarded(h, [&](pscbfm::Updater& u) {
+        if (!pairs && n_pairs) throw std::invalid_argument("null pair array");
+        u.setConnectivityBulk(pairs, n_pairs);
+    });
+}
+
+int pscbfm_set_lattice_size(pscbfm_updater* h, uint32_t x, uint32_t y, uint32_t z) {
+    return guarded(h, [&](pscbfm::Updater& u) { u.setLatticeSize(x, y, z); });
+}
+
+int pscbfm_set_periodicity(pscbfm_updater* h, int x, int y, int z) {
+    return guarded(h, [&](pscbfm::Updater& u) { u.setPeriodicity(x != 0, y != 0, z != 0); });
+}
+
+int pscbfm_populate_lattice(pscbfm_updater* h) {
+    return guarded(h, [](pscbfm::Updater& u) { u.populateLattice(); });
+}
+
+int pscbfm_initialize(pscbfm_updater* h) {
+    return guarded(h, [](pscbfm::Updater& u) { u.initialize(); });
+}
+
+int pscbfm_execute(pscbfm_updater* h) { return h ? PSCBFM_OK : PSCBFM_NULL_HANDLE; }
+
+int pscbfm_run(pscbfm_updater* h, int32_t n_mcs, uint64_t* accepted_out) {
+    return guarded(h, [&](pscbfm::Updater& u) {
+        u.runSimulationOnGPU(n_mcs);
+        if (accepted_out) *accepted_out = u.lastRun().accepted;
+    });
+}
+
+int pscbfm_get_monomer_position(const pscbfm_updater* h, uint32_t i, int32_t* x, int32_t* y, int32_t* z) {
+    return guarded_const(h, [&](const pscbfm::Updater& u) {
+        const int32_t vx = u.getMonomerPositionInX(i);
+        const int32_t vy = u.getMonomerPositionInY(i);
+        const int32_t vz = u.getMonomerPositionInZ(i);
+        if (x) *x = vx;
+        if (y) *y = vy;
+        if (z) *z = vz;
+    });
+}
+
+int pscbfm_get_positions_bulk(const pscbfm_updater* h, int32_t* xyz, uint32_t n) {
+    return guarded_const(h, [&](const pscbfm::Updater& u) {
+        if (!xyz && n) throw std::invalid_argument("null coordinate array");
+        u.getPositionsBulk(xyz, n);
+    });
+}
+
+int pscbfm_check_system(const pscbfm_updater* h) {
+    return guarded_const(h, [](const pscbfm::Updater& u) { u.checkSystem(); });
+}
+
+int pscbfm_cleanup(pscbfm_updater* h) {
+    return guarded(h, [](pscbfm::Updater& u) { u.cleanup(); });
+}
+
+int pscbfm_set_seed(pscbfm_updater* h, uint64_t seed) {
+    return guarded(h, [&](pscbfm::Updater& u) { u.setSeed(seed); });
+}
+
+int pscbfm_set_lattice_kind(pscbfm_updater* h, int kind) {
+    return guarded(h, [&](pscbfm::Updater& u) {
+        if (kind < 0) throw std::invalid_argument("[setLatticeKind] unknown kind");
+        u.setLatticeKind(uint32_t(kind));
+    });
+}
+
+int pscbfm_set_uniform_coloring(pscbfm_updater* h, int on) {
+    return guarded(h, [&](pscbfm::Updater& u) { u.setUniformColoring(on != 0); });
+}
+
+int pscbfm_set_check_on_initialize(pscbfm_updater* h, int on) {
+    return guarded(h, [&](pscbfm::Updater& u) { u.setCheckOnInitialize(on != 0); });
+}
+
+int pscbfm_set_colors(pscbfm_updater* h, const uint8_t* colors, uint32_t n) {
+    return guarded(h, [&](pscbfm::Updater& u) {
+        if (!colors && n) throw std::invalid_argument("null color array");
+        u.setColors(colors, n);
+    });
+}
+
+int pscbfm_set_launch_shape(pscbfm_updater* h, uint32_t block, uint32_t check_cap, uint32_t perform_cap) {
+    return guarded(h, [&](pscbfm::Updater& u) { u.setLaunchShape(block, check_cap, perform_cap); });
+}
+
+int pscbfm_set_graph_chunk(pscbfm_updater* h, uint32_t substeps) {
+    return guarded(h, [&](pscbfm::Updater& u) { u.setGraphChunk(substeps); });
+}
+
+int pscbfm_run_ex(pscbfm_updater* h, int32_t n_mcs, uint32_t* log, int profile, int use_graph,
+                  pscbfm_run_stats* stats) {
+    return guarded(h, [&](pscbfm::Updater& u) {
+        u.run(n_mcs, log, profile != 0, use_graph != 0);
+        if (stats) fill_stats(u.lastRun(), stats);
+    });
+}
+
+int pscbfm_last_run_stats(const pscbfm_updater* h, pscbfm_run_stats* stats) {
+    return guarded_const(h, [&](const pscbfm::Updater& u) {
+        if (!stats) throw std::invalid_argument("null stats");
+        fill_stats(u.lastRun(), stats);
+    });
+}
+
+int pscbfm_get_n_colors(const pscbfm_updater* h, uint32_t* n_colors) {
+    return guarded_const(h, [&](const pscbfm::Updater& u) {
+        if (!u.initialized()) throw std::runtime_error("not initialized");
+        if (n_colors) *n_colors = u.nColors();
+    });
+}
+
+int pscbfm_get_colors(const pscbfm_updater* h, uint8_t* colors, uint32_t n) {
+    return guarded_const(h, [&](const pscbfm::Updater& u) {
+        if (!u.initialized()) throw std::runtime_error("not initialized");
+        if (n != u.nMonomers()) throw std::invalid_argument("count differs from the number of monomers");
+        std::memcpy(colors, u.layout().color.data(), n);
+    });
+}
+
+int pscbfm_get_color_sizes(const pscbfm_updater* h, uint32_t* sizes, uint32_t n_colors) {
+    return guarded_const(h, [&](const pscbfm::Updater& u) {
+        if (!u.initialized()) throw std::runtime_error("not initialized");
+        if (n_colors != u.nColors()) throw std::invalid_argument("wrong number of colors");
+        for (uint32_t c = 0; c < n_colors; ++c) sizes[c] = u.layout().colors[c].count;
+    });
+}
+
+int pscbfm_get_counters(const pscbfm_updater* h, uint64_t* substeps, uint64_t* accepted) {
+    return guarded_const(h, [&](const pscbfm::Updater& u) {
+        if (substeps) *substeps = u.substepCounter();
+        if (accepted) *accepted = u.acceptedTotal();
+    });
+}
+
+int pscbfm_get_device_bytes(const pscbfm_updater* h, uint64_t* bytes) {
+    return guarded_const(h, [&](const pscbfm::Updater& u) {
+        if (bytes) *bytes = u.deviceBytes();
+    });
+}
+
+int pscbfm_download_lattice(const pscbfm_updater* h, uint8_t* cells, uint64_t n_cells) {
+    return guarded_const(h, [&](const pscbfm::Updater& u) {
+        std::vector<uint8_t> lat;
+        u.downloadLattice(lat);
+        if (n_cells != lat.size()) throw std::invalid_argument("wrong number of cells");
+        std::memcpy(cells, lat.data(), lat.size());
+    });
+}
+
+int pscbfm_count_proposal_bits(const pscbfm_updater* h, uint64_t* bits_even, uint64_t* bits_odd) {
+    return guarded_const(h, [&](const pscbfm::Updater& u) {
+        std::vector<uint32_t> w;
+        u.downloadProposal(w);
+        const size_t half = w.size() / 2;
+        uint64_t a = 0, b = 0;
+        for (size_t i = 0; i < half; ++i) a += uint64_t(__builtin_popcount(w[i]));
+        for (size_t i = half; i < w.size(); ++i) b += uint64_t(__builtin_popcount(w[i]));
+        if (bits_even) *bits_even = a;
+        if (bits_odd) *bits_odd = b;
+    });
+}
+
+int pscbfm_zcurve_encode(uint32_t lx, uint32_t ly, uint32_t lz, uint32_t x, uint32_t y, uint32_t z,
+                         uint32_t* code) {
+    if (!code) return PSCBFM_INVALID_ARGUMENT;
+    pscbfm::MortonCodec c;
+    c.init(lx, ly, lz);
+    *code = c.encode(x & (lx - 1), y & (ly - 1), z & (lz - 1));
+    return PSCBFM_OK;
+}
+
+uint32_t pscbfm_move_hash(uint32_t a) { return pscbfm::move_hash(a); }
+
+void pscbfm_substep_draw(uint64_t seed, uint64_t t, uint32_t n_colors, uint32_t* species, uint32_t* kernel_seed) {
+    const pscbfm::SubstepDraw d = pscbfm::substep_draw(seed, t, n_colors ? n_colors : 1u);
+    if (species) *species = d.species;
+    if (kernel_seed) *kernel_seed = d.seed;
+}
+
+int pscbfm_color_graph(uint32_t n, const uint32_t* start, const uint32_t* ids, int uniform, uint8_t* colors) {
+    try {
+        pscbfm::Adjacency adj;
+        adj.start.assign(start, start + n + 1);
+        adj.ids.assign(ids, ids + start[n]);
+        for (uint32_t id : adj.ids)
+            if (id >= n) return PSCBFM_OUT_OF_RANGE;
+        const std::vector<uint8_t> c = pscbfm::color_graph(adj, uniform != 0);
+        std::memcpy(colors, c.data(), n);
+        return PSCBFM_OK;
+    } catch (const std::runtime_error&) {
+        return PSCBFM_RUNTIME_ERROR;
+    } catch (...) {
+        return PSCBFM_INTERNAL_ERROR;
+    }
+}
+
+int pscbfm_standard_bond_vectors(int32_t* xyz108) {
+    if (!xyz108) return PSCBFM_INVALID_ARGUMENT;
+    const auto& v = pscbfm::standard_bond_vectors();
+    for (size_t i = 0; i < v.size(); ++i)
+        for (int a = 0; a < 3; ++a) xyz108[3 * i + a] = v[i][size_t(a)];
+    return v.size() == 108 ? PSCBFM_OK : PSCBFM_INTERNAL_ERROR;
+}
+
+namespace {
+bool valid_box(uint32_t lx, uint32_t ly, uint32_t lz) {
+    auto p2 = [](uint32_t v) { return v >= 8 && (v & (v - 1)) == 0; };
+    return p2(lx) && p2(ly) && p2(lz) && uint64_t(lx) * ly * lz <= (uint64_t(1) << 32);
+}
+}  // namespace
+
+int pscbfm_gen_linear_chains(uint32_t lx, uint32_t ly, uint32_t lz, uint32_t n_chains, uint32_t len,
+                             uint64_t seed, uint32_t max_restarts, int32_t* xyz) {
+    if (!valid_box(lx, ly, lz) || (!xyz && n_chains && len)) return PSCBFM_INVALID_ARGUMENT;
+    try {
+        const uint32_t box[3] = {lx, ly, lz};
+        std::vector<int32_t> out;
+        if (!pscbfm::grow_linear_chains(box, n_chains, len, seed, max_restarts, out)) return PSCBFM_RUNTIME_ERROR;
+        std::memcpy(xyz, out.data(), out.size() * sizeof(int32_t));
+        return PSCBFM_OK;
+    } catch (...) {
+        return PSCBFM_INTERNAL_ERROR;
+    }
+}
+
+int pscbfm_gen_linear_chains_ordered(uint32_t lx, uint32_t ly, uint32_t lz, uint32_t n_chains, uint32_t len,
+                                     int32_t* xyz) {
+    if (!valid_box(lx, ly, lz) || (!xyz && n_chains && len)) return PSCBFM_INVALID_ARGUMENT;
+    try {
+        const uint32_t box[3] = {lx, ly, lz};
+        std::vector<int32_t> out;
+        if (!pscbfm::place_linear_chains_ordered(box, n_chains, len, out)) return PSCBFM_RUNTIME_ERROR;
+        std::memcpy(xyz, out.data(), out.size() * sizeof(int32_t));
+        return PSCBFM_OK;
+    } catch (...) {
+        return PSCBFM_INTERNAL_ERROR;
+    }
+}
+
+int pscbfm_gen_star_network(uint32_t lx, uint32_t ly, uint32_t lz, uint32_t n_stars, uint32_t arms,
+                            uint32_t arm_length, uint64_t seed, int32_t* xyz, uint32_t* bonds,
+                            uint64_t bond_capacity, uint64_t* n_bonds) {
+    if (!valid_box(lx, ly, lz) || !xyz || !bonds || !n_bonds) return PSCBFM_INVALID_ARGUMENT;
+    try {
+        const uint32_t box[3] = {lx, ly, lz};
+        std::vector<int32_t> out;
+        std::vector<uint32_t> b;
+        if (!pscbfm::build_star_network(box, n_stars, arms, arm_length, seed, out, b)) return PSCBFM_RUNTIME_ERROR;
+        if (b.size() / 2 > bond_capacity) return PSCBFM_OUT_OF_RANGE;
+        std::memcpy(xyz, out.data(), out.size() * sizeof(int32_t));
+        std::memcpy(bonds, b.data(), b.size() * sizeof(uint32_t));
+        *n_bonds = b.size() / 2;
+        return PSCBFM_OK;
+    } catch (...) {
+        return PSCBFM_INTERNAL_ERROR;
+    }
+}
+
+const char* pscbfm_version(void) { return "pscbfm-mi355x 0.1 (gfx950)"; }
+
+}  // extern "C"

@@ -1,0 +1,52 @@
+// Host-side layout of one system for the sweep kernels: species-sorted slots,
+// per-species neighbour matrices, Z-curve positions and image counters.
+#pragma once
+
+#include <cstdint>
+#include <vector>
+
+#include "coloring.h"
+#include "common.h"
+
+namespace pscbfm {
+
+struct SystemInput {
+    uint32_t box[3] = {0, 0, 0};
+    std::vector<int32_t> xyz;        // unfolded corner, 3 per monomer
+    Adjacency bonds;                 // directed bond lists as registered
+    uint8_t forbidden[512];          // reference bond table, index (dx&7)|(dy&7)<<3|(dz&7)<<6
+};
+
+struct SystemLayout {
+    MortonCodec codec;
+    uint32_t nMonomers = 0;
+    uint32_t nSlots = 0;                 // padded slot count
+    std::vector<uint8_t> color;          // per monomer
+    std::vector<ColorInfo> colors;       // per species
+    std::vector<uint32_t> slotOf;        // monomer -> slot
+    std::vector<uint32_t> monomerOf;     // slot -> monomer (UINT32_MAX for padding)
+    std::vector<uint32_t> pos;           // per slot, Z-curve code
+    std::vector<int32_t> img[3];         // per slot, periodic images
+    std::vector<uint32_t> nbr;           // neighbour matrices (slot ids)
+    uint32_t bondMask[16];               // Z-order bond keys, bit set = forbidden
+    uint32_t maxColorCount = 0;
+
+    // Unfolded coordinates of all monomers in original order.
+    void unfolded(const std::vector<uint32_t>& posSlots, const std::vector<int32_t>* imgSlots,
+                  int32_t* xyzOut) const;
+};
+
+// Floor division / modulo by a power-of-two box edge.
+inline int32_t floor_div_pow2(int32_t v, uint32_t log2L) {
+    return v >> log2L;  // arithmetic shift is floor division for two's complement
+}
+
+// Builds the layout.  `colorOverride` (optional) replaces the automatic
+// coloring; it must be a proper coloring.
+SystemLayout build_layout(const SystemInput& in, const std::vector<uint8_t>* colorOverride,
+                          bool uniformColors);
+
+// Bond key (Z-order, see bond_key in common.h) -> reference table index.
+uint32_t bond_key_to_reference_index(uint32_t key);
+
+}  // namespace pscbfm

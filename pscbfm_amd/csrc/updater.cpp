@@ -1,0 +1,644 @@
+// Host engine of the scBFM sweep (see updater.h).
+#include "updater.h"
+
+#include <algorithm>
+#include <cstdlib>
+#include <cstring>
+#include <limits>
+#include <sstream>
+#include <stdexcept>
+
+namespace pscbfm {
+
+namespace {
+
+[[noreturn]] void hip_fail(hipError_t e, const char* what) {
+    std::ostringstream msg;
+    msg << "HIP error in " << what << ": " << hipGetErrorName(e) << " (" << hipGetErrorString(e) << ")";
+    throw std::runtime_error(msg.str());
+}
+
+inline void hip_check(hipError_t e, const char* what) {
+    if (e != hipSuccess) hip_fail(e, what);
+}
+
+bool is_pow2(uint32_t v) { return v != 0 && (v & (v - 1)) == 0; }
+
+uint64_t align_up(uint64_t v, uint64_t a) { return (v + a - 1) / a * a; }
+
+}  // namespace
+
+Updater::Updater() { std::memset(forbidden_, 0, sizeof forbidden_); }
+
+Updater::~Updater() {
+    try {
+        freeDevice();
+    } catch (...) {
+    }
+}
+
+// UpdaterGPUScBFM_AB_Type.cu:472-487
+void Updater::setGpu(int id) {
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess) count = 0;
+    if (id < 0 || id >= count) {
+        std::ostringstream msg;
+        msg << "[UpdaterGPUScBFM_AB_Type::setGpu] GPU with ID " << id << " not present. Only " << count
+            << " GPUs are available.";
+        throw std::invalid_argument(msg.str());
+    }
+    hip_check(hipSetDevice(id), "hipSetDevice");
+    gpu_ = id;
+}
+
+// UpdaterGPUScBFM_AB_Type.cu:693-697 (table index of CRITICAL.cu:243-261)
+void Updater::copyBondSet(int dx, int dy, int dz, bool forbidden) {
+    forbidden_[(uint32_t(dx) & 7u) | ((uint32_t(dy) & 7u) << 3) | ((uint32_t(dz) & 7u) << 6)] = forbidden ? 1 : 0;
+}
+
+// UpdaterGPUScBFM_AB_Type.cu:699-726
+void Updater::setNrOfAllMonomers(uint32_t n) {
+    if (n_ != 0 || initialized_) {
+        std::ostringstream msg;
+        msg << "[UpdaterGPUScBFM_AB_Type::setNrOfAllMonomers] Number of Monomers already set to " << n_ << "!";
+        throw std::runtime_error(msg.str());
+    }
+    if (n > (1u << 28))
+        throw std::invalid_argument("[UpdaterGPUScBFM_AB_Type::setNrOfAllMonomers] at most 2^28 monomers are supported");
+    n_ = n;
+    nSet_ = true;
+    xyz_.assign(3 * size_t(n), 0);
+    attr_.assign(n, 0);
+    degree_.assign(n, 0);
+    nbrIds_.assign(size_t(n) * kMaxConnectivity, 0);
+}
+
+void Updater::requireMonomers(const char* who) const {
+    if (!nSet_) {
+        std::ostringstream msg;
+        msg << "[UpdaterGPUScBFM_AB_Type::" << who << "] setNrOfAllMonomers has not been called";
+        throw std::runtime_error(msg.str());
+    }
+}
+
+void Updater::requireIndex(uint32_t i, const char* who) const {
+    if (i >= n_) {
+        std::ostringstream msg;
+        msg << "[UpdaterGPUScBFM_AB_Type::" << who << "] monomer index " << i << " out of range (" << n_
+            << " monomers)";
+        throw std::out_of_range(msg.str());
+    }
+}
+
+// UpdaterGPUScBFM_AB_Type.cu:755-758 (stored, not used by athermal dynamics)
+void Updater::setAttribute(uint32_t i, int32_t a) {
+    requireMonomers("setAttribute");
+    requireIndex(i, "setAttribute");
+    attr_[i] = a;
+}
+
+// UpdaterGPUScBFM_AB_Type.cu:760-771
+void Updater::setMonomerCoordinates(uint32_t i, int32_t x, int32_t y, int32_t z) {
+    requireMonomers("setMonomerCoordinates");
+    requireIndex(i, "setMonomerCoordinates");
+    if (initialized_)
+        throw std::runtime_error(
+            "[UpdaterGPUScBFM_AB_Type::setMonomerCoordinates] coordinates are owned by the device after initialize");
+    xyz_[3 * size_t(i) + 0] = x;
+    xyz_[3 * size_t(i) + 1] = y;
+    xyz_[3 * size_t(i) + 2] = z;
+}
+
+void Updater::setMonomerCoordinatesBulk(const int32_t* xyz, uint32_t n) {
+    requireMonomers("setMonomerCoordinatesBulk");
+    if (n != n_) throw std::invalid_argument("[setMonomerCoordinatesBulk] count differs from setNrOfAllMonomers");
+    if (initialized_) throw std::runtime_error("[setMonomerCoordinatesBulk] already initialized");
+    std::memcpy(xyz_.data(), xyz, sizeof(int32_t) * 3 * size_t(n));
+}
+
+// UpdaterGPUScBFM_AB_Type.cu:777-795 (directed: callers register both ways)
+void Updater::setConnectivity(uint32_t i, uint32_t j) {
+    requireMonomers("setConnectivity");
+    requireIndex(i, "setConnectivity");
+    requireIndex(j, "setConnectivity");
+    if (initialized_) throw std::runtime_error("[UpdaterGPUScBFM_AB_Type::setConnectivity] already initialized");
+    if (degree_[i] >= kMaxConnectivity) {
+        std::ostringstream msg;
+        msg << "[UpdaterGPUScBFM_AB_Type::setConnectivity] The maximum amount of bonds per monomer ("
+            << kMaxConnectivity << ") has been exceeded!";
+        throw std::invalid_argument(msg.str());
+    }
+    nbrIds_[size_t(i) * kMaxConnectivity + degree_[i]++] = j;
+}
+
+void Updater::setConnectivityBulk(const uint32_t* pairs, uint64_t nPairs) {
+    for (uint64_t p = 0; p < nPairs; ++p) setConnectivity(pairs[2 * p], pairs[2 * p + 1]);
+}
+
+// UpdaterGPUScBFM_AB_Type.cu:797-848
+void Updater::setLatticeSize(uint32_t x, uint32_t y, uint32_t z) {
+    if (box_[0] == x && box_[1] == y && box_[2] == z) return;
+    if (initialized_) throw std::runtime_error("[UpdaterGPUScBFM_AB_Type::setLatticeSize] already initialized");
+    const uint32_t lim = uint32_t(std::numeric_limits<int16_t>::max());
+    if (x > lim || y > lim || z > lim) {
+        std::ostringstream msg;
+        msg << "[UpdaterGPUScBFM_AB_Type::setLatticeSize] The box size (" << x << "," << y << "," << z
+            << ") is larger than the internal integer data type for representing positions allow! (-32768 <= "
+               "size <= 32767)";
+        throw std::invalid_argument(msg.str());
+    }
+    if (!is_pow2(x) || !is_pow2(y) || !is_pow2(z)) {
+        std::ostringstream msg;
+        msg << "[UpdaterGPUScBFM_AB_Type::setLatticeSize] Check whether the box size is a power of 2! (" << x
+            << "," << y << "," << z << ")";
+        throw std::runtime_error(msg.str());
+    }
+    if (x < 8 || y < 8 || z < 8)
+        throw std::runtime_error("[UpdaterGPUScBFM_AB_Type::setLatticeSize] every box edge must be at least 8");
+    if (uint64_t(x) * y * z > (uint64_t(1) << 32))
+        throw std::runtime_error("[UpdaterGPUScBFM_AB_Type::setLatticeSize] at most 2^32 lattice cells");
+    box_[0] = x;
+    box_[1] = y;
+    box_[2] = z;
+    hostLattice_.clear();
+}
+
+// UpdaterGPUScBFM_AB_Type.cu:728-753
+void Updater::setPeriodicity(bool x, bool y, bool z) {
+    if (!x || !y || !z) {
+        std::ostringstream msg;
+        msg << "[UpdaterGPUScBFM_AB_Type::setPeriodicity] Simulation is intended to use completely periodic "
+               "boundary conditions, but setPeriodicity was called with ("
+            << x << "," << y << "," << z << ")";
+        throw std::invalid_argument(msg.str());
+    }
+}
+
+// UpdaterGPUScBFM_AB_Type.cu:850-872: corner occupancy on the host.
+void Updater::populateLattice() {
+    requireMonomers("populateLattice");
+    if (box_[0] == 0) throw std::runtime_error("[UpdaterGPUScBFM_AB_Type::populateLattice] setLatticeSize first");
+    MortonCodec codec;
+    codec.init(box_[0], box_[1], box_[2]);
+    hostLattice_.assign(size_t(box_[0]) * box_[1] * box_[2], 0);
+    for (uint32_t i = 0; i < n_; ++i) {
+        const uint32_t x = uint32_t(xyz_[3 * size_t(i)]) & (box_[0] - 1);
+        const uint32_t y = uint32_t(xyz_[3 * size_t(i) + 1]) & (box_[1] - 1);
+        const uint32_t z = uint32_t(xyz_[3 * size_t(i) + 2]) & (box_[2] - 1);
+        hostLattice_[codec.encode(x, y, z)] = 1;
+    }
+}
+
+void Updater::setSeed(uint64_t seed) {
+    if (initialized_) throw std::runtime_error("[setSeed] the seed is fixed at initialize");
+    seed_ = seed;
+}
+
+void Updater::setLatticeKind(uint32_t kind) {
+    if (initialized_) throw std::runtime_error("[setLatticeKind] the lattice kind is fixed at initialize");
+    if (kind != kLatticeU8 && kind != kLatticeBit) throw std::invalid_argument("[setLatticeKind] unknown kind");
+    latticeKind_ = kind;
+}
+
+void Updater::setColors(const uint8_t* colors, uint32_t n) {
+    requireMonomers("setColors");
+    if (n != n_) throw std::invalid_argument("[setColors] count differs from setNrOfAllMonomers");
+    colorOverride_.assign(colors, colors + n);
+    hasColorOverride_ = true;
+}
+
+void Updater::setLaunchShape(uint32_t block, uint32_t checkCap, uint32_t performCap) {
+    if (block != 0 && (block % 64 != 0 || block > 1024))
+        throw std::invalid_argument("[setLaunchShape] block must be a multiple of 64 up to 1024");
+    blockOverride_ = block;
+    checkCapOverride_ = checkCap;
+    performCapOverride_ = performCap;
+    if (initialized_) {
+        const uint32_t b = block ? block : 256;
+        shape_.block = b;
+        shape_.checkGrid = grid_for(layout_.maxColorCount, b, checkCap ? checkCap : (1u << 20));
+        shape_.performGrid = grid_for(layout_.maxColorCount, b, performCap ? performCap : 4096u);
+        if (graphExec_) {
+            (void)hipGraphExecDestroy(graphExec_);
+            graphExec_ = nullptr;
+        }
+    }
+}
+
+// UpdaterGPUScBFM_AB_Type.cu:882-986: excluded volume (8 cells per monomer,
+// no overlap) and every bond in the set.
+void Updater::checkSystem() const {
+    requireMonomers("checkSystem");
+    if (box_[0] == 0) throw std::runtime_error("[UpdaterGPUScBFM_AB_Type::checkSystem] setLatticeSize first");
+    const uint64_t cells = uint64_t(box_[0]) * box_[1] * box_[2];
+    std::vector<uint8_t> occ(cells, 0);
+    for (uint32_t i = 0; i < n_; ++i) {
+        for (uint32_t c = 0; c < 8; ++c) {
+            const uint32_t x = uint32_t(xyz_[3 * size_t(i)] + int32_t(c & 1u)) & (box_[0] - 1);
+            const uint32_t y = uint32_t(xyz_[3 * size_t(i) + 1] + int32_t((c >> 1) & 1u)) & (box_[1] - 1);
+            const uint32_t z = uint32_t(xyz_[3 * size_t(i) + 2] + int32_t((c >> 2) & 1u)) & (box_[2] - 1);
+            const uint64_t lin = x + uint64_t(box_[0]) * (y + uint64_t(box_[1]) * z);
+            if (occ[lin]) {
+                std::ostringstream msg;
+                msg << "[UpdaterGPUScBFM_AB_Type::checkSystem] excluded volume violated: monomer " << i
+                    << " at (" << xyz_[3 * size_t(i)] << "," << xyz_[3 * size_t(i) + 1] << ","
+                    << xyz_[3 * size_t(i) + 2] << ") overlaps another monomer";
+                throw std::runtime_error(msg.str());
+            }
+            occ[lin] = 1;
+        }
+    }
+    for (uint32_t i = 0; i < n_; ++i) {
+        for (uint32_t k = 0; k < degree_[i]; ++k) {
+            const uint32_t j = nbrIds_[size_t(i) * kMaxConnectivity + k];
+            // Minimum image: a network bond can join monomers whose unfolded
+            // coordinates differ by whole box lengths; the sweep only uses the
+            // vector modulo 8, which every box edge is a multiple of.
+            int32_t d[3];
+            for (int a = 0; a < 3; ++a) {
+                const int32_t L = int32_t(box_[a]);
+                const int32_t raw = xyz_[3 * size_t(j) + a] - xyz_[3 * size_t(i) + a];
+                d[a] = int32_t(uint32_t(raw + L / 2) & uint32_t(L - 1)) - L / 2;
+            }
+            const bool tooLong = std::abs(d[0]) > 3 || std::abs(d[1]) > 3 || std::abs(d[2]) > 3;
+            const uint32_t idx =
+                (uint32_t(d[0]) & 7u) | ((uint32_t(d[1]) & 7u) << 3) | ((uint32_t(d[2]) & 7u) << 6);
+            if (tooLong || forbidden_[idx]) {
+                std::ostringstream msg;
+                msg << "[UpdaterGPUScBFM_AB_Type::checkSystem] forbidden bond (" << d[0] << "," << d[1] << ","
+                    << d[2] << ") between monomer " << i + 1 << " and monomer " << j + 1;
+                throw std::runtime_error(msg.str());
+            }
+        }
+    }
+}
+
+void Updater::freeDevice() {
+    if (graphExec_) {
+        (void)hipGraphExecDestroy(graphExec_);
+        graphExec_ = nullptr;
+    }
+    if (arena_) {
+        (void)hipFree(arena_);
+        arena_ = nullptr;
+    }
+    if (stream_) {
+        (void)hipStreamDestroy(stream_);
+        stream_ = nullptr;
+    }
+    arenaBytes_ = 0;
+}
+
+// UpdaterGPUScBFM_AB_Type.cu:490-690
+void Updater::initialize() {
+    if (initialized_) {
+        throw std::runtime_error(
+            "[UpdaterGPUScBFM_AB_Type::initialize] Initialize was already called and may not be called again until "
+            "cleanup was called!");
+    }
+    requireMonomers("initialize");
+    if (box_[0] == 0) throw std::runtime_error("[UpdaterGPUScBFM_AB_Type::initialize] setLatticeSize first");
+    unsigned nAllowed = 0;
+    for (int i = 0; i < 512; ++i) nAllowed += forbidden_[i] ? 0u : 1u;
+    if (nAllowed != 108) {
+        std::ostringstream msg;
+        msg << "[UpdaterGPUScBFM_AB_Type::initialize] Wrong bond-set! Expected 108 allowed bonds, but got "
+            << nAllowed;
+        throw std::runtime_error(msg.str());
+    }
+    if (checkOnInit_) checkSystem();
+
+    SystemInput in;
+    std::memcpy(in.box, box_, sizeof box_);
+    in.xyz = xyz_;
+    in.bonds.start.assign(size_t(n_) + 1, 0);
+    for (uint32_t i = 0; i < n_; ++i) in.bonds.start[i + 1] = in.bonds.start[i] + degree_[i];
+    in.bonds.ids.resize(in.bonds.start[n_]);
+    for (uint32_t i = 0; i < n_; ++i)
+        for (uint32_t k = 0; k < degree_[i]; ++k)
+            in.bonds.ids[in.bonds.start[i] + k] = nbrIds_[size_t(i) * kMaxConnectivity + k];
+    std::memcpy(in.forbidden, forbidden_, sizeof forbidden_);
+    layout_ = build_layout(in, hasColorOverride_ ? &colorOverride_ : nullptr, uniformColors_);
+
+    if (gpu_ < 0) {
+        int dev = 0;
+        hip_check(hipGetDevice(&dev), "hipGetDevice");
+        gpu_ = dev;
+    } else {
+        hip_check(hipSetDevice(gpu_), "hipSetDevice");
+    }
+    hip_check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
+
+    cells_ = uint64_t(box_[0]) * box_[1] * box_[2];
+    const uint64_t latWords = cells_ / 32;
+    const uint64_t nSlots = std::max<uint64_t>(layout_.nSlots, 1);
+    struct Piece {
+        uint64_t bytes;
+        uint64_t off;
+    };
+    enum { P_POS, P_IMGX, P_IMGY, P_IMGZ, P_FLAGS, P_NBR, P_LAT, P_PROP, P_COLORS, P_BOND, P_STATE, P_N };
+    Piece p[P_N];
+    p[P_POS].bytes = nSlots * 4;
+    p[P_IMGX].bytes = nSlots * 4;
+    p[P_IMGY].bytes = nSlots * 4;
+    p[P_IMGZ].bytes = nSlots * 4;
+    p[P_FLAGS].bytes = nSlots;
+    p[P_NBR].bytes = std::max<uint64_t>(layout_.nbr.size(), 1) * 4;
+    p[P_LAT].bytes = latticeKind_ == kLatticeU8 ? cells_ : latWords * 4;
+    p[P_PROP].bytes = 2 * latWords * 4;
+    p[P_COLORS].bytes = std::max<uint64_t>(layout_.colors.size(), 1) * sizeof(ColorInfo);
+    p[P_BOND].bytes = 64;
+    p[P_STATE].bytes = sizeof(DevState);
+    uint64_t total = 0;
+    for (int i = 0; i < P_N; ++i) {
+        p[i].off = total;
+        total = align_up(total + p[i].bytes, 256);
+    }
+    hip_check(hipMalloc(&arena_, total), "hipMalloc");
+    arenaBytes_ = total;
+    char* base = static_cast<char*>(arena_);
+
+    desc_ = SysDesc{};
+    desc_.pos = reinterpret_cast<uint32_t*>(base + p[P_POS].off);
+    desc_.imgX = reinterpret_cast<int32_t*>(base + p[P_IMGX].off);
+    desc_.imgY = reinterpret_cast<int32_t*>(base + p[P_IMGY].off);
+    desc_.imgZ = reinterpret_cast<int32_t*>(base + p[P_IMGZ].off);
+    desc_.flags = reinterpret_cast<uint8_t*>(base + p[P_FLAGS].off);
+    desc_.nbr = reinterpret_cast<const uint32_t*>(base + p[P_NBR].off);
+    desc_.lattice = base + p[P_LAT].off;
+    desc_.proposal = reinterpret_cast<uint32_t*>(base + p[P_PROP].off);
+    desc_.colors = reinterpret_cast<const ColorInfo*>(base + p[P_COLORS].off);
+    desc_.bondMask = reinterpret_cast<const uint32_t*>(base + p[P_BOND].off);
+    desc_.state = reinterpret_cast<DevState*>(base + p[P_STATE].off);
+    desc_.seed = seed_;
+    for (int a = 0; a < 3; ++a) desc_.mask[a] = layout_.codec.mask[a];
+    desc_.nColors = uint32_t(layout_.colors.size());
+    desc_.latWords = uint32_t(latWords);
+    desc_.latticeKind = latticeKind_;
+    desc_.maxColorCount = layout_.maxColorCount;
+
+    // Host images of every device buffer, uploaded once.
+    hip_check(hipMemsetAsync(arena_, 0, total, stream_), "hipMemsetAsync");
+    auto up = [&](int piece, const void* src, uint64_t bytes) {
+        if (bytes) hip_check(hipMemcpyAsync(base + p[piece].off, src, bytes, hipMemcpyHostToDevice, stream_),
+                             "hipMemcpyAsync H2D");
+    };
+    up(P_POS, layout_.pos.data(), uint64_t(layout_.nSlots) * 4);
+    up(P_IMGX, layout_.img[0].data(), uint64_t(layout_.nSlots) * 4);
+    up(P_IMGY, layout_.img[1].data(), uint64_t(layout_.nSlots) * 4);
+    up(P_IMGZ, layout_.img[2].data(), uint64_t(layout_.nSlots) * 4);
+    up(P_NBR, layout_.nbr.data(), uint64_t(layout_.nbr.size()) * 4);
+    up(P_COLORS, layout_.colors.data(), uint64_t(layout_.colors.size()) * sizeof(ColorInfo));
+    up(P_BOND, layout_.bondMask, 64);
+    {
+        // Occupancy lattice in Z-curve order: byte per cell or bit per cell.
+        if (latticeKind_ == kLatticeU8) {
+            std::vector<uint8_t> lat(cells_, 0);
+            for (uint32_t i = 0; i < n_; ++i) lat[layout_.pos[layout_.slotOf[i]]] = 1;
+            up(P_LAT, lat.data(), cells_);
+        } else {
+            std::vector<uint32_t> lat(latWords, 0);
+            for (uint32_t i = 0; i < n_; ++i) {
+                const uint32_t c = layout_.pos[layout_.slotOf[i]];
+                lat[c >> 5] |= 1u << (c & 31u);
+            }
+            up(P_LAT, lat.data(), latWords * 4);
+        }
+    }
+    DevState st{};
+    st.substep = 0;
+    st.accepted = 0;
+    st.log = nullptr;
+    st.stopAt = std::numeric_limits<uint64_t>::max();
+    up(P_STATE, &st, sizeof st);
+    hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize (initialize)");
+
+    const uint32_t b = blockOverride_ ? blockOverride_ : 256;
+    shape_.block = b;
+    shape_.checkGrid = grid_for(layout_.maxColorCount, b, checkCapOverride_ ? checkCapOverride_ : (1u << 20));
+    shape_.performGrid = grid_for(layout_.maxColorCount, b, performCapOverride_ ? performCapOverride_ : 4096u);
+    substep_ = 0;
+    acceptedTotal_ = 0;
+    initialized_ = true;
+}
+
+void Updater::captureGraph() {
+    if (graphExec_) {
+        (void)hipGraphExecDestroy(graphExec_);
+        graphExec_ = nullptr;
+    }
+    const uint32_t nc = nColors();
+    uint32_t g = graphChunk_;
+    if (g == 0) g = nc * std::max<uint32_t>(1u, 32u / std::max<uint32_t>(nc, 1u));
+    graphSubsteps_ = g;
+    hipGraph_t graph = nullptr;
+    hip_check(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+    hipError_t err = hipSuccess;
+    for (uint32_t k = 0; k < g && err == hipSuccess; ++k) {
+        err = launch_check(desc_, k, shape_, stream_);
+        if (err == hipSuccess) err = launch_perform(desc_, k, shape_, stream_);
+    }
+    if (err == hipSuccess) err = launch_advance(desc_.state, g, stream_);
+    const hipError_t endErr = hipStreamEndCapture(stream_, &graph);
+    if (err != hipSuccess) hip_fail(err, "kernel launch during capture");
+    hip_check(endErr, "hipStreamEndCapture");
+    const hipError_t instErr = hipGraphInstantiate(&graphExec_, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    hip_check(instErr, "hipGraphInstantiate");
+}
+
+void Updater::run(int32_t nMCS, uint32_t* log, bool profile, bool useGraph) {
+    if (!initialized_)
+        throw std::runtime_error("[UpdaterGPUScBFM_AB_Type::runSimulationOnGPU] initialize has not been called");
+    lastRun_ = RunStats{};
+    if (nMCS <= 0) return;
+    hip_check(hipSetDevice(gpu_), "hipSetDevice");
+    const uint32_t nc = nColors();
+    if (nc == 0 || n_ == 0) return;
+    const uint64_t S = uint64_t(nMCS) * nc;
+
+    uint32_t* dLog = nullptr;
+    if (log) {
+        hip_check(hipMalloc(&dLog, S * 4), "hipMalloc (log)");
+        hip_check(launch_fill_u32(dLog, S, 0u, stream_), "fill log");
+        DevState patch{};
+        hip_check(hipMemcpyAsync(&patch, desc_.state, sizeof patch, hipMemcpyDeviceToHost, stream_), "state D2H");
+        hip_check(hipStreamSynchronize(stream_), "sync");
+        patch.log = dLog;
+        patch.logFirst = substep_;
+        patch.logCapacity = S;
+        hip_check(hipMemcpyAsync(desc_.state, &patch, sizeof patch, hipMemcpyHostToDevice, stream_), "state H2D");
+    }
+
+    hipEvent_t e0, e1;
+    hip_check(hipEventCreate(&e0), "hipEventCreate");
+    hip_check(hipEventCreate(&e1), "hipEventCreate");
+    std::vector<hipEvent_t> kev;
+    hip_check(hipEventRecord(e0, stream_), "hipEventRecord");
+    if (profile) {
+        kev.resize(3 * S);
+        for (auto& e : kev) hip_check(hipEventCreate(&e), "hipEventCreate");
+        for (uint64_t k = 0; k < S; ++k) {
+            hip_check(hipEventRecord(kev[3 * k], stream_), "hipEventRecord");
+            hip_check(launch_check(desc_, 0, shape_, stream_), "launch pass 1");
+            hip_check(hipEventRecord(kev[3 * k + 1], stream_), "hipEventRecord");
+            hip_check(launch_perform(desc_, 0, shape_, stream_), "launch pass 2");
+            hip_check(hipEventRecord(kev[3 * k + 2], stream_), "hipEventRecord");
+            hip_check(launch_advance(desc_.state, 1, stream_), "launch advance");
+        }
+    } else {
+        uint64_t done = 0;
+        if (useGraph) {
+            if (!graphExec_) captureGraph();
+            while (S - done >= graphSubsteps_) {
+                hip_check(hipGraphLaunch(graphExec_, stream_), "hipGraphLaunch");
+                done += graphSubsteps_;
+            }
+        }
+        const uint64_t rem = S - done;
+        // Eager tail in blocks of at most 2^31 substeps per advance.
+        uint64_t left = rem;
+        while (left > 0) {
+            const uint32_t blk = uint32_t(std::min<uint64_t>(left, 1u << 20));
+            for (uint32_t k = 0; k < blk; ++k) {
+                hip_check(launch_check(desc_, k, shape_, stream_), "launch pass 1");
+                hip_check(launch_perform(desc_, k, shape_, stream_), "launch pass 2");
+            }
+            hip_check(launch_advance(desc_.state, blk, stream_), "launch advance");
+            left -= blk;
+        }
+    }
+    hip_check(hipEventRecord(e1, stream_), "hipEventRecord");
+    hip_check(hipEventSynchronize(e1), "hipEventSynchronize");
+    float ms = 0;
+    hip_check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+    lastRun_.gpuMilliseconds = ms;
+    if (profile) {
+        for (uint64_t k = 0; k < S; ++k) {
+            float a = 0, b = 0;
+            hip_check(hipEventElapsedTime(&a, kev[3 * k], kev[3 * k + 1]), "hipEventElapsedTime");
+            hip_check(hipEventElapsedTime(&b, kev[3 * k + 1], kev[3 * k + 2]), "hipEventElapsedTime");
+            lastRun_.checkMilliseconds += a;
+            lastRun_.performMilliseconds += b;
+        }
+        lastRun_.checkLaunches = S;
+        lastRun_.performLaunches = S;
+        for (auto& e : kev) (void)hipEventDestroy(e);
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+
+    // Attempts and bond terms follow from the host replica of the draws.
+    for (uint64_t k = 0; k < S; ++k) {
+        const SubstepDraw d = substep_draw(seed_, substep_ + k, nc);
+        const ColorInfo& ci = layout_.colors[d.species];
+        lastRun_.attempted += ci.count;
+    }
+    {
+        // Bond terms per species: total degree of its members.
+        std::vector<uint64_t> deg(nc, 0);
+        for (uint32_t i = 0; i < n_; ++i) deg[layout_.color[i]] += degree_[i];
+        for (uint64_t k = 0; k < S; ++k) lastRun_.bondTerms += deg[substep_draw(seed_, substep_ + k, nc).species];
+    }
+    lastRun_.substeps = S;
+    substep_ += S;
+
+    if (log) {
+        hip_check(hipMemcpy(log, dLog, S * 4, hipMemcpyDeviceToHost), "log D2H");
+        DevState patch{};
+        hip_check(hipMemcpy(&patch, desc_.state, sizeof patch, hipMemcpyDeviceToHost), "state D2H");
+        patch.log = nullptr;
+        patch.logCapacity = 0;
+        hip_check(hipMemcpy(desc_.state, &patch, sizeof patch, hipMemcpyHostToDevice), "state H2D");
+        (void)hipFree(dLog);
+    }
+    syncFromDevice();
+    lastRun_.accepted = 0;
+    {
+        DevState st{};
+        hip_check(hipMemcpy(&st, desc_.state, sizeof st, hipMemcpyDeviceToHost), "state D2H");
+        lastRun_.accepted = st.accepted - acceptedTotal_;
+        acceptedTotal_ = st.accepted;
+        if (st.substep != substep_) throw std::runtime_error("device substep counter out of step with the host");
+    }
+}
+
+// CRITICAL.cu:414-698 (synchronous; returns after copy-back and unsort)
+void Updater::runSimulationOnGPU(int32_t nMCS) { run(nMCS, nullptr, false, true); }
+
+void Updater::syncFromDevice() {
+    const uint32_t ns = layout_.nSlots;
+    std::vector<uint32_t> pos(ns);
+    std::vector<int32_t> img[3];
+    for (auto& v : img) v.resize(ns);
+    if (ns) {
+        hip_check(hipMemcpy(pos.data(), desc_.pos, size_t(ns) * 4, hipMemcpyDeviceToHost), "pos D2H");
+        hip_check(hipMemcpy(img[0].data(), desc_.imgX, size_t(ns) * 4, hipMemcpyDeviceToHost), "img D2H");
+        hip_check(hipMemcpy(img[1].data(), desc_.imgY, size_t(ns) * 4, hipMemcpyDeviceToHost), "img D2H");
+        hip_check(hipMemcpy(img[2].data(), desc_.imgZ, size_t(ns) * 4, hipMemcpyDeviceToHost), "img D2H");
+    }
+    layout_.unfolded(pos, img, xyz_.data());
+}
+
+void Updater::downloadLattice(std::vector<uint8_t>& out) const {
+    if (!initialized_) throw std::runtime_error("[downloadLattice] not initialized");
+    out.assign(cells_, 0);
+    if (latticeKind_ == kLatticeU8) {
+        hip_check(hipMemcpy(out.data(), desc_.lattice, cells_, hipMemcpyDeviceToHost), "lattice D2H");
+    } else {
+        std::vector<uint32_t> w(cells_ / 32);
+        hip_check(hipMemcpy(w.data(), desc_.lattice, w.size() * 4, hipMemcpyDeviceToHost), "lattice D2H");
+        for (uint64_t c = 0; c < cells_; ++c) out[c] = (w[c >> 5] >> (c & 31u)) & 1u;
+    }
+}
+
+void Updater::downloadProposal(std::vector<uint32_t>& out) const {
+    if (!initialized_) throw std::runtime_error("[downloadProposal] not initialized");
+    out.resize(2 * (cells_ / 32));
+    hip_check(hipMemcpy(out.data(), desc_.proposal, out.size() * 4, hipMemcpyDeviceToHost), "proposal D2H");
+}
+
+int32_t Updater::getMonomerPositionInX(uint32_t i) const {
+    requireIndex(i, "getMonomerPositionInX");
+    return xyz_[3 * size_t(i)];
+}
+int32_t Updater::getMonomerPositionInY(uint32_t i) const {
+    requireIndex(i, "getMonomerPositionInY");
+    return xyz_[3 * size_t(i) + 1];
+}
+int32_t Updater::getMonomerPositionInZ(uint32_t i) const {
+    requireIndex(i, "getMonomerPositionInZ");
+    return xyz_[3 * size_t(i) + 2];
+}
+
+void Updater::getPositionsBulk(int32_t* xyz, uint32_t n) const {
+    if (n != n_) throw std::invalid_argument("[getPositionsBulk] count differs from the number of monomers");
+    std::memcpy(xyz, xyz_.data(), sizeof(int32_t) * 3 * size_t(n));
+}
+
+// UpdaterGPUScBFM_AB_Type.cu:455-465.  Unlike the reference this also forgets
+// the monomer count, so the setter sequence can be replayed after cleanup.
+void Updater::destruct() {
+    freeDevice();
+    initialized_ = false;
+    layout_ = SystemLayout{};
+    n_ = 0;
+    nSet_ = false;
+    xyz_.clear();
+    attr_.clear();
+    degree_.clear();
+    nbrIds_.clear();
+    hostLattice_.clear();
+    colorOverride_.clear();
+    hasColorOverride_ = false;
+    substep_ = 0;
+    acceptedTotal_ = 0;
+}
+
+// UpdaterGPUScBFM_AB_Type.cu:995-1001
+void Updater::cleanup() {
+    destruct();
+    if (gpu_ >= 0) (void)hipDeviceSynchronize();
+}
+
+}  // namespace pscbfm

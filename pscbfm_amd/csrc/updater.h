@@ -1,0 +1,137 @@
+// Host engine behind the UpdaterGPUScBFM_AB_Type surface: staging of the
+// reference's setter calls, species layout, device buffers, the captured
+// substep graph and the synchronous sweep driver.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "common.h"
+#include "kernels.h"
+#include "layout.h"
+
+namespace pscbfm {
+
+struct RunStats {
+    double gpuMilliseconds = 0;     // hipEvent time of the launched substeps
+    uint64_t substeps = 0;
+    uint64_t attempted = 0;         // sum of species sizes over the substeps
+    uint64_t accepted = 0;
+    uint64_t bondTerms = 0;         // sum of bonds of the drawn species
+    // Per-kernel durations, only filled by a profiled run.
+    double checkMilliseconds = 0;
+    double performMilliseconds = 0;
+    uint64_t checkLaunches = 0;
+    uint64_t performLaunches = 0;
+};
+
+class Updater {
+public:
+    Updater();
+    ~Updater();
+    Updater(const Updater&) = delete;
+    Updater& operator=(const Updater&) = delete;
+
+    // --- reference surface (UpdaterGPUScBFM_AB_Type.h:316-361) ------------
+    void setGpu(int id);
+    void copyBondSet(int dx, int dy, int dz, bool forbidden);
+    void setNrOfAllMonomers(uint32_t n);
+    void setAttribute(uint32_t i, int32_t a);
+    void setMonomerCoordinates(uint32_t i, int32_t x, int32_t y, int32_t z);
+    void setConnectivity(uint32_t i, uint32_t j);
+    void setLatticeSize(uint32_t x, uint32_t y, uint32_t z);
+    void setPeriodicity(bool x, bool y, bool z);
+    void populateLattice();
+    void initialize();
+    void runSimulationOnGPU(int32_t nMCS);
+    int32_t getMonomerPositionInX(uint32_t i) const;
+    int32_t getMonomerPositionInY(uint32_t i) const;
+    int32_t getMonomerPositionInZ(uint32_t i) const;
+    void checkSystem() const;
+    void cleanup();
+    void destruct();
+
+    // --- extensions ---------------------------------------------------------
+    void setSeed(uint64_t seed);
+    void setLatticeKind(uint32_t kind);
+    void setUniformColoring(bool on) { uniformColors_ = on; }
+    void setCheckOnInitialize(bool on) { checkOnInit_ = on; }
+    void setColors(const uint8_t* colors, uint32_t n);
+    void setLaunchShape(uint32_t block, uint32_t checkCap, uint32_t performCap);
+    void setGraphChunk(uint32_t substeps) { graphChunk_ = substeps; }
+    void setMonomerCoordinatesBulk(const int32_t* xyz, uint32_t n);
+    void setConnectivityBulk(const uint32_t* pairs, uint64_t nPairs);
+    void getPositionsBulk(int32_t* xyz, uint32_t n) const;
+    // Runs nMCS sweeps; `log` (if not null) receives nMCS*nColors accepted
+    // counts, one per substep.  profile=true launches eagerly with an event
+    // pair around every kernel to time pass 1 and pass 2 separately.
+    void run(int32_t nMCS, uint32_t* log, bool profile, bool useGraph);
+
+    uint32_t nMonomers() const { return n_; }
+    uint32_t nColors() const { return uint32_t(layout_.colors.size()); }
+    const SystemLayout& layout() const { return layout_; }
+    const RunStats& lastRun() const { return lastRun_; }
+    uint64_t substepCounter() const { return substep_; }
+    uint64_t acceptedTotal() const { return acceptedTotal_; }
+    uint64_t deviceBytes() const { return arenaBytes_; }
+    bool initialized() const { return initialized_; }
+    uint32_t latticeKind() const { return latticeKind_; }
+    const LaunchShape& launchShape() const { return shape_; }
+    const uint32_t* box() const { return box_; }
+    int gpu() const { return gpu_; }
+    uint64_t seed() const { return seed_; }
+    // Host view of the occupancy lattice (one byte per cell, Z-curve order)
+    // as built by populateLattice().
+    const std::vector<uint8_t>& hostLattice() const { return hostLattice_; }
+    // Copies the device occupancy lattice to `out` (cells bytes, 0/1).
+    void downloadLattice(std::vector<uint8_t>& out) const;
+    // Copies both proposal lattices (2 * cells / 8 bytes).
+    void downloadProposal(std::vector<uint32_t>& out) const;
+
+private:
+    void requireMonomers(const char* who) const;
+    void requireIndex(uint32_t i, const char* who) const;
+    void freeDevice();
+    void captureGraph();
+    void syncFromDevice();
+
+    // staging
+    int gpu_ = -1;
+    uint32_t n_ = 0;
+    bool nSet_ = false;
+    std::vector<int32_t> xyz_;
+    std::vector<int32_t> attr_;
+    std::vector<uint32_t> degree_;
+    std::vector<uint32_t> nbrIds_;   // kMaxConnectivity per monomer
+    uint32_t box_[3] = {0, 0, 0};
+    uint8_t forbidden_[512];
+    std::vector<uint8_t> hostLattice_;
+    std::vector<uint8_t> colorOverride_;
+    bool hasColorOverride_ = false;
+    uint64_t seed_ = 0;
+    uint32_t latticeKind_ = kLatticeBit;
+    bool uniformColors_ = true;
+    bool checkOnInit_ = true;
+    uint32_t graphChunk_ = 0;   // 0: choose automatically
+    uint32_t blockOverride_ = 0, checkCapOverride_ = 0, performCapOverride_ = 0;
+
+    // device
+    bool initialized_ = false;
+    SystemLayout layout_;
+    hipStream_t stream_ = nullptr;
+    void* arena_ = nullptr;
+    uint64_t arenaBytes_ = 0;
+    SysDesc desc_{};
+    LaunchShape shape_;
+    hipGraphExec_t graphExec_ = nullptr;
+    uint32_t graphSubsteps_ = 0;
+    uint64_t substep_ = 0;
+    uint64_t acceptedTotal_ = 0;
+    uint64_t cells_ = 0;
+    RunStats lastRun_;
+};
+
+}  // namespace pscbfm

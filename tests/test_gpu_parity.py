@@ -1,0 +1,85 @@
+"""GPU parity: the HIP sweep against the CPU oracle, bit for bit.
+
+Every case compares final unfolded coordinates and the accepted count of
+every substep.  Sizes are chosen so the oracle finishes in seconds.
+"""
+import numpy as np
+import pytest
+
+import pscbfm_amd as P
+from helpers import chains, gpu_vs_oracle, make_updater, ordered_chains, stars, Sys
+
+pytestmark = pytest.mark.gpu
+
+KINDS = [P.LATTICE_BIT, P.LATTICE_U8]
+
+
+@pytest.mark.parametrize("kind", KINDS)
+def test_small_chains_bit_exact(kind):
+    s = chains((32, 32, 32), 20, 16, seed=7)
+    gx, glog, ox, olog = gpu_vs_oracle(s, seed=42, n_mcs=50, kind=kind)
+    assert np.array_equal(glog, olog)
+    assert np.array_equal(gx, ox)
+    assert glog.sum() > 0
+
+
+@pytest.mark.parametrize("kind", KINDS)
+def test_c1_shape_reduced_mcs(kind):
+    # Config "plain single system": 64 chains of 1024 on 256^3, seed 42, at 20 MCS.
+    s = chains((256, 256, 256), 64, 1024, seed=42)
+    gx, glog, ox, olog = gpu_vs_oracle(s, seed=42, n_mcs=20, kind=kind)
+    assert np.array_equal(glog, olog)
+    assert np.array_equal(gx, ox)
+
+
+@pytest.mark.parametrize("kind", KINDS)
+def test_dense_ordered_melt(kind):
+    # phi = 0.5 ordered packing, the start state of the dense-melt workload, small box.
+    box = (64, 64, 64)
+    n_chains = 64 * 64 * 64 // 16 // 32
+    s = ordered_chains(box, n_chains, 32)
+    gx, glog, ox, olog = gpu_vs_oracle(s, seed=3, n_mcs=30, kind=kind)
+    assert np.array_equal(glog, olog)
+    assert np.array_equal(gx, ox)
+
+
+def test_split_calls_and_graph_off_equal_one_call():
+    s = chains((64, 64, 64), 40, 64, seed=11)
+    a = gpu_vs_oracle(s, seed=5, n_mcs=24, chunks=(24,))
+    b = gpu_vs_oracle(s, seed=5, n_mcs=24, chunks=(1, 7, 16), use_graph=False)
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+    assert np.array_equal(a[0], a[2]) and np.array_equal(a[1], a[3])
+
+
+def test_star_network_multicolor():
+    s = stars((64, 64, 64), 40, 4, 29, seed=9)
+    gx, glog, ox, olog = gpu_vs_oracle(s, seed=77, n_mcs=20)
+    assert np.array_equal(glog, olog)
+    assert np.array_equal(gx, ox)
+
+
+@pytest.mark.parametrize("block", [64, 512, 1024])
+def test_launch_shape_does_not_change_results(block):
+    s = chains((64, 64, 64), 30, 40, seed=2)
+    gx, glog, ox, olog = gpu_vs_oracle(s, seed=8, n_mcs=10, block=block)
+    assert np.array_equal(glog, olog)
+    assert np.array_equal(gx, ox)
+
+
+def test_invariants_after_run():
+    s = chains((64, 64, 64), 50, 50, seed=4)
+    u = make_updater(s, seed=1)
+    u.runSimulationOnGPU(25)
+    u.checkSystem()
+    # occupancy lattice holds exactly one set cell per monomer
+    lat = u.downloadLattice(64 ** 3)
+    assert int(lat.sum()) == s.n
+    u.cleanup()
+
+
+def test_single_monomer_always_moves():
+    s = Sys((16, 16, 16), np.array([[3, 4, 5]], np.int32), np.zeros((0, 2), np.uint32))
+    u = make_updater(s, seed=0)
+    stats, log = u.run(100, log=True)
+    assert stats.accepted == 100 and np.all(log == 1)
+    u.cleanup()
