@@ -57,7 +57,7 @@ def standard_forbidden() -> np.ndarray:
     return forb
 
 
-def make_updater(box, xyz, bonds, seed, kind):
+def make_updater(box, xyz, bonds, seed, kind, block=0, slots=0):
     u = P.UpdaterGPUScBFM_AB_Type()
     u.setGpu(0)
     u.setPeriodicity(True, True, True)
@@ -73,6 +73,8 @@ def make_updater(box, xyz, bonds, seed, kind):
     u.setSeed(seed)
     u.setLatticeKind(kind)
     u.setCheckOnInitialize(False)
+    if block or slots:
+        u.setLaunchShape(block, slots)
     u.initialize()
     return u, forb
 
@@ -141,6 +143,8 @@ def main() -> int:
     ap.add_argument("--chain", type=int, default=256)
     ap.add_argument("--phi", type=float, default=0.5)
     ap.add_argument("--seed", type=int, default=42)
+    ap.add_argument("--block", type=int, default=0, help="threads per workgroup (0: default)")
+    ap.add_argument("--slots", type=int, default=0, help="monomers per pass-1 thread (0: default)")
     ap.add_argument("--profile-mcs", type=int, default=20)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -162,7 +166,7 @@ def main() -> int:
     tic = time.perf_counter()
     xyz = P.gen_linear_chains_ordered(box, n_chains, args.chain)
     bonds = P.chain_bonds(n_chains, args.chain)
-    u, _ = make_updater(box, xyz, bonds, args.seed, kind)
+    u, _ = make_updater(box, xyz, bonds, args.seed, kind, args.block, args.slots)
     colors = u.colors(xyz.shape[0])
     log(f"initialize done in {time.perf_counter() - tic:.1f} s; species sizes {u.colorSizes().tolist()}; "
         f"device bytes {u.deviceBytes() / 2**20:.0f} MiB")

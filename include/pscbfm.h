@@ -102,9 +102,9 @@ int pscbfm_set_uniform_coloring(pscbfm_updater* h, int on);
 int pscbfm_set_check_on_initialize(pscbfm_updater* h, int on);
 /* Replace the automatic coloring with a proper coloring given per monomer. */
 int pscbfm_set_colors(pscbfm_updater* h, const uint8_t* colors, uint32_t n);
-/* Threads per workgroup (multiple of 64, 0 = 256) and workgroup caps of the
- * two kernels (0 = default). */
-int pscbfm_set_launch_shape(pscbfm_updater* h, uint32_t block, uint32_t check_cap, uint32_t perform_cap);
+/* Threads per workgroup (64, 128, 192 or 256; 0 = 256) and monomers per
+ * pass-1 thread (1, 2 or 4; 0 = 2).  Results do not depend on the shape. */
+int pscbfm_set_launch_shape(pscbfm_updater* h, uint32_t block, uint32_t slots);
 /* Substeps per captured graph (0 = automatic). */
 int pscbfm_set_graph_chunk(pscbfm_updater* h, uint32_t substeps);
 

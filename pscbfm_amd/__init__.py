@@ -90,7 +90,7 @@ _SIGNATURES = {
     "pscbfm_set_uniform_coloring": (C.c_int, [_H, C.c_int]),
     "pscbfm_set_check_on_initialize": (C.c_int, [_H, C.c_int]),
     "pscbfm_set_colors": (C.c_int, [_H, _u8p, C.c_uint32]),
-    "pscbfm_set_launch_shape": (C.c_int, [_H, C.c_uint32, C.c_uint32, C.c_uint32]),
+    "pscbfm_set_launch_shape": (C.c_int, [_H, C.c_uint32, C.c_uint32]),
     "pscbfm_set_graph_chunk": (C.c_int, [_H, C.c_uint32]),
     "pscbfm_run_ex": (C.c_int, [_H, C.c_int32, _u32p, C.c_int, C.c_int, C.POINTER(RunStats)]),
     "pscbfm_last_run_stats": (C.c_int, [_H, C.POINTER(RunStats)]),
@@ -247,8 +247,8 @@ class UpdaterGPUScBFM_AB_Type:
         arr = np.ascontiguousarray(colors, dtype=np.uint8)
         self._call("pscbfm_set_colors", _ptr(arr, C.c_uint8), int(arr.size))
 
-    def setLaunchShape(self, block: int = 0, check_cap: int = 0, perform_cap: int = 0) -> None:
-        self._call("pscbfm_set_launch_shape", int(block), int(check_cap), int(perform_cap))
+    def setLaunchShape(self, block: int = 0, slots: int = 0) -> None:
+        self._call("pscbfm_set_launch_shape", int(block), int(slots))
 
     def setGraphChunk(self, substeps: int) -> None:
         self._call("pscbfm_set_graph_chunk", int(substeps))

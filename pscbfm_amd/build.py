@@ -16,6 +16,7 @@ from pathlib import Path
 
 HERE = Path(__file__).resolve().parent
 CSRC = HERE / "csrc"
+INCLUDE = HERE.parent / "include"
 OUT = HERE / "libpscbfm.so"
 BUILD = HERE / "_build"
 ARCH = os.environ.get("PSCBFM_OFFLOAD_ARCH", "gfx950")
@@ -38,7 +39,7 @@ def _hipcc() -> str:
 def _compile(src: Path) -> Path:
     obj = BUILD / (src.name + ".o")
     flags = DEVICE if src.suffix == ".hip" else HOST
-    cmd = [_hipcc(), *COMMON, *flags, "-I", str(CSRC), "-c", str(src), "-o", str(obj)]
+    cmd = [_hipcc(), *COMMON, *flags, "-I", str(CSRC), "-I", str(INCLUDE), "-c", str(src), "-o", str(obj)]
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         raise RuntimeError(f"compile failed: {' '.join(cmd)}\n{res.stdout}\n{res.stderr}")

@@ -1,6 +1,6 @@
 // extern "C" boundary of the scBFM sweep (include/pscbfm.h).  Exceptions of
 // the engine become status codes plus a per-handle message.
-#include "../../include/pscbfm.h"
+#include "pscbfm.h"
 
 #include <cstring>
 #include <new>
@@ -208,8 +208,8 @@ int pscbfm_set_colors(pscbfm_updater* h, const uint8_t* colors, uint32_t n) {
     });
 }
 
-int pscbfm_set_launch_shape(pscbfm_updater* h, uint32_t block, uint32_t check_cap, uint32_t perform_cap) {
-    return guarded(h, [&](pscbfm::Updater& u) { u.setLaunchShape(block, check_cap, perform_cap); });
+int pscbfm_set_launch_shape(pscbfm_updater* h, uint32_t block, uint32_t slots) {
+    return guarded(h, [&](pscbfm::Updater& u) { u.setLaunchShape(block, slots); });
 }
 
 int pscbfm_set_graph_chunk(pscbfm_updater* h, uint32_t substeps) {

@@ -10,6 +10,8 @@
 // the own corner is the stored value itself.
 #pragma once
 
+#include <hip/hip_runtime.h>
+
 #include <cstdint>
 
 #if defined(__HIPCC__)
@@ -178,15 +180,22 @@ struct ColorInfo {
     uint32_t pad_[3];
 };
 
+// Accepted moves are not counted with atomics: workgroup w of the pass-2
+// kernel adds its count to its own running total acceptedByBlock[w] (one
+// writer per entry, kernels ordered on the stream), and the host sums the
+// entries.  Thousands of workgroups adding to one address would serialise on
+// a single memory channel.
+constexpr uint32_t kMaxPerformBlocks = 1u << 16;
+
 // Mutable per-system state that lives on the device so that a captured
 // graph can replay any number of substeps without host involvement.
 struct DevState {
-    uint64_t substep;        // global substep counter t of the next substep
-    uint64_t accepted;       // accepted moves since initialize
-    uint32_t* log;           // optional per-substep accepted counts
-    uint64_t logFirst;       // substep index stored in log[0]
-    uint64_t logCapacity;    // entries in log
-    uint64_t stopAt;         // substeps >= stopAt are skipped (batched runs)
+    uint64_t substep;          // global substep counter t of the next substep
+    uint64_t stopAt;           // substeps >= stopAt are skipped (batched runs)
+    uint32_t* log;             // optional per-substep accepted counts
+    uint64_t logFirst;         // substep index stored in log[0]
+    uint64_t logCapacity;      // entries in log
+    uint64_t* acceptedByBlock; // kMaxPerformBlocks running totals
     uint64_t pad_[2];
 };
 
@@ -196,20 +205,21 @@ struct SysDesc {
     int32_t* imgX;           // periodic image counters per slot
     int32_t* imgY;
     int32_t* imgZ;
-    uint8_t* flags;          // pass-1 verdict: 0 or 1 + direction
     const uint32_t* nbr;     // neighbour matrices (slot ids)
     void* lattice;           // uint8 per cell or 1 bit per cell
     uint32_t* proposal;      // two 1-bit proposal lattices, latWords each
     const ColorInfo* colors; // nColors entries
     const uint32_t* bondMask;// 16 words: bit k set <=> Z-order bond key k forbidden
     DevState* state;
+    uint2* list;             // pass-1 survivors, one list per pass-1 workgroup
+    uint32_t* listCount;     // entries of each list
     uint64_t seed;           // key of the substep draw
     uint32_t mask[3];        // Z-curve bit masks of x, y, z
     uint32_t nColors;
     uint32_t latWords;       // 32-bit words of one 1-bit lattice (cells / 32)
     uint32_t latticeKind;
     uint32_t maxColorCount;  // largest species
-    uint32_t pad_;
+    uint32_t maxDegree;      // largest bond count of any monomer
 };
 
 }  // namespace pscbfm

@@ -7,10 +7,13 @@
 
 namespace pscbfm {
 
+// Both passes run on the same grid: workgroup w of pass 1 covers species
+// indices [w*slots*block, (w+1)*slots*block) and writes list w, which
+// workgroup w of pass 2 consumes.
 struct LaunchShape {
-    uint32_t block = 256;       // threads per workgroup
-    uint32_t checkGrid = 8;     // workgroups of the pass-1 kernel (multiple of 8)
-    uint32_t performGrid = 8;   // workgroups of the pass-2 kernel (multiple of 8)
+    uint32_t block = 256;  // threads per workgroup (64..256, multiple of 64)
+    uint32_t slots = 2;    // monomers per pass-1 thread (1, 2 or 4)
+    uint32_t grid = 8;     // workgroups (multiple of 8)
 };
 
 // One substep = pass 1 (propose/check) then pass 2 (resolve/apply).  `k` is
@@ -23,14 +26,17 @@ hipError_t launch_advance(DevState* state, uint32_t count, hipStream_t st);
 // Fills n 32-bit words with v (graph-safe replacement of a memset).
 hipError_t launch_fill_u32(uint32_t* p, uint64_t n, uint32_t v, hipStream_t st);
 
-// Grid shape that covers `items` with `block` threads per workgroup,
-// rounded up to whole XCD rounds and capped at `cap` workgroups.
-inline uint32_t grid_for(uint64_t items, uint32_t block, uint32_t cap) {
-    uint64_t g = (items + block - 1) / block;
+// Workgroups that cover `items` with `perGroup` items each, rounded up to
+// whole XCD rounds (multiples of 8).
+inline uint32_t grid_for(uint64_t items, uint32_t perGroup) {
+    uint64_t g = (items + perGroup - 1) / perGroup;
     if (g < 1) g = 1;
-    if (g > cap) g = cap;
     g = (g + 7) / 8 * 8;
     return uint32_t(g);
 }
+// List entries needed for any shape with slots*block <= 1024.
+inline uint64_t list_capacity(uint64_t maxColorCount) { return maxColorCount + 8u * 1024u; }
+// Workgroups of the finest shape (slots*block = 64).
+inline uint64_t max_grid(uint64_t maxColorCount) { return grid_for(maxColorCount, 64); }
 
 }  // namespace pscbfm

@@ -9,6 +9,15 @@
 //    and every neighbouring cell is a masked add on one axis' bits;
 //  * the forbidden-bond table is a 512-bit mask in LDS, indexed by the
 //    Z-order bond key of two codes;
+//  * both passes are bound by memory latency, not bandwidth: a pass-1 thread
+//    handles SLOTS monomers and issues all their loads in two rounds (own
+//    codes and partner ids; then partner codes and the nine front-plane
+//    cells together), so each round trip carries SLOTS times more requests;
+//  * pass 1 compacts the surviving proposals of its workgroup into a list
+//    (wave ballot + LDS counter; the order inside a list is irrelevant
+//    because the result of a substep does not depend on processing order),
+//    and pass 2 walks those lists densely: one load of (index, direction,
+//    code), one round of nine proposal-lattice loads, then the stores;
 //  * the proposal lattice is 1 bit per cell and double-buffered by substep
 //    parity: pass 1 of substep t sets bits in buffer t&1 and zeroes buffer
 //    (t+1)&1, which pass 2 of substep t-1 was the last to read;
@@ -32,7 +41,6 @@ __device__ __forceinline__ uint32_t xcd_block() {
 struct Move {
     uint32_t to;        // code of the new corner
     uint32_t cell[9];   // the 3x3 plane in front of the move
-    uint32_t axisMask;
     bool wraps;         // the axis coordinate crosses the periodic boundary
 };
 
@@ -49,7 +57,6 @@ __device__ __forceinline__ Move make_move(uint32_t m, uint32_t dir, const uint32
     Move mv;
     const uint32_t qa = dil_add(m, Ma, stepA);
     mv.to = (m & ~Ma) | qa;
-    mv.axisMask = Ma;
     mv.wraps = plus ? ((m & Ma) == Ma) : ((m & Ma) == 0u);
     const uint32_t fa = dil_add(qa, Ma, stepA);
     const uint32_t u0 = m & Mu, v0 = m & Mv;
@@ -63,32 +70,46 @@ __device__ __forceinline__ Move make_move(uint32_t m, uint32_t dir, const uint32
     return mv;
 }
 
-template <int LAT>
-__device__ __forceinline__ bool front_occupied(const void* lattice, const Move& mv) {
-    uint32_t any = 0;
-    if (LAT == kLatticeU8) {
-        const uint8_t* lat = static_cast<const uint8_t*>(lattice);
+// Whether any of the nine front-plane cells is set, read as 64-bit words.
+// The low code bits interleave x, y and z, so a 64-bit word of the 1-bit
+// lattice (64 cells, SHIFT 6) is an aligned 4x4x4 block and one of the byte
+// lattice (8 cells, SHIFT 3) an aligned 2x2x2 block.  In either case the 3x3
+// plane touches at most four words, those of (u-1 or u+1) x (v-1 or v+1):
+// four 8-byte gathers replace nine narrow ones, and the middle row/column
+// takes the word of whichever outer row/column shares its block.
+template <int SHIFT>
+__device__ __forceinline__ uint32_t plane_any(const uint64_t* words, const Move& mv) {
+    const uint64_t w00 = words[mv.cell[0] >> SHIFT];  // (u-1, v-1)
+    const uint64_t w02 = words[mv.cell[2] >> SHIFT];  // (u-1, v+1)
+    const uint64_t w20 = words[mv.cell[6] >> SHIFT];  // (u+1, v-1)
+    const uint64_t w22 = words[mv.cell[8] >> SHIFT];  // (u+1, v+1)
+    // does the middle u (v) share the block of u-1 (v-1)?
+    const bool uLow = ((mv.cell[3] ^ mv.cell[0]) >> SHIFT) == 0u;
+    const bool vLow = ((mv.cell[1] ^ mv.cell[0]) >> SHIFT) == 0u;
+    const uint64_t wRow0 = vLow ? w00 : w02, wRow2 = vLow ? w20 : w22;  // words of (u-1, v), (u+1, v)
+    const uint64_t w10 = uLow ? w00 : w20, w12 = uLow ? w02 : w22;      // words of (u, v-1), (u, v+1)
+    const uint64_t w11 = uLow ? wRow0 : wRow2;                          // word of (u, v)
+    const uint64_t w[9] = {w00, wRow0, w02, w10, w11, w12, w20, wRow2, w22};
+    constexpr uint32_t kLow = (1u << SHIFT) - 1u;
+    constexpr uint32_t kUnit = SHIFT == 6 ? 1u : 8u;  // bits per cell inside the word
+    uint64_t any = 0;
 #pragma unroll
-        for (int c = 0; c < 9; ++c) any |= lat[mv.cell[c]];
-    } else {
-        const uint32_t* lat = static_cast<const uint32_t*>(lattice);
-#pragma unroll
-        for (int c = 0; c < 9; ++c) any |= lat[mv.cell[c] >> 5] >> (mv.cell[c] & 31u);
-        any &= 1u;
-    }
-    return any != 0u;
+    for (int c = 0; c < 9; ++c) any |= w[c] >> ((mv.cell[c] & kLow) * kUnit);
+    return uint32_t(any & (SHIFT == 6 ? 1u : 0xFFu));
 }
 
-__device__ __forceinline__ bool front_proposed(const uint32_t* T, const Move& mv) {
-    uint32_t any = 0;
-#pragma unroll
-    for (int c = 0; c < 9; ++c) any |= T[mv.cell[c] >> 5] >> (mv.cell[c] & 31u);
-    return (any & 1u) != 0u;
+// Rank of this lane among the lanes of its wave whose bit is set in `mask`.
+__device__ __forceinline__ uint32_t lane_rank(uint64_t mask) {
+    return __builtin_amdgcn_mbcnt_hi(uint32_t(mask >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(mask), 0u));
 }
 
-template <int LAT>
-__global__ __launch_bounds__(1024) void k_check(SysDesc d, uint32_t k) {
+// List entry: x = species-local index | direction << 29, y = old code.
+constexpr uint32_t kDirShift = 29;
+
+template <int LAT, int SLOTS, int MAXDEG>
+__global__ __launch_bounds__(256) void k_check(SysDesc d, uint32_t k) {
     __shared__ uint32_t sForbidden[16];
+    __shared__ uint32_t sCount;
     const DevState* st = d.state;
     const uint64_t t = st->substep + k;
     if (t >= st->stopAt) return;
@@ -107,32 +128,75 @@ __global__ __launch_bounds__(1024) void k_check(SysDesc d, uint32_t k) {
         for (uint32_t q = q0 + threadIdx.x; q < q1; q += blockDim.x) prev[q] = make_uint4(0u, 0u, 0u, 0u);
     }
     if (threadIdx.x < 16) sForbidden[threadIdx.x] = d.bondMask[threadIdx.x];
+    if (threadIdx.x == 0) sCount = 0;
     __syncthreads();
 
-    uint32_t* T = d.proposal + size_t(t & 1u) * d.latWords;
-    const uint32_t* nbr = d.nbr + ci.nbrOffset;
-    for (uint32_t j = lb * blockDim.x + threadIdx.x; j < ci.count; j += nb * blockDim.x) {
-        const uint32_t slot = ci.slotOffset + j;
-        const uint32_t m = d.pos[slot];
-        // CRITICAL.cu:322-323: the draw is keyed by the species-local index.
-        const uint32_t dir = move_hash(move_hash(j) ^ draw.seed) % 6u;
-        const Move mv = make_move(m, dir, d.mask);
-        bool ok = true;
-        for (uint32_t r = 0; r < ci.maxDegree; ++r) {
-            const uint32_t partner = nbr[size_t(r) * ci.nbrPitch + j];
-            if (partner != kNoNeighbor) {
-                const uint32_t key = bond_key(d.pos[partner], mv.to);
-                ok = ok && !((sForbidden[key >> 5] >> (key & 31u)) & 1u);
-            }
+    const uint32_t B = blockDim.x;
+    const uint32_t base = lb * (SLOTS * B);
+    const uint32_t* pos = d.pos + ci.slotOffset;
+
+    // The load phase is branch-free: lanes without a monomer (or without a
+    // partner in a row) load from an in-bounds stand-in address and discard
+    // the value, so every load of a thread sits in one basic block and all of
+    // them are in flight together.
+    // Round 1: own codes and partner ids.
+    uint32_t j[SLOTS], m[SLOTS], partner[SLOTS][MAXDEG];
+    bool live[SLOTS];
+#pragma unroll
+    for (int s = 0; s < SLOTS; ++s) {
+        j[s] = base + uint32_t(s) * B + threadIdx.x;
+        live[s] = j[s] < ci.count;
+        const uint32_t jj = live[s] ? j[s] : 0u;  // count > 0 for every species
+        m[s] = pos[jj];
+#pragma unroll
+        for (int r = 0; r < MAXDEG; ++r) {
+            const bool row = uint32_t(r) < ci.maxDegree;
+            const uint32_t id = d.nbr[row ? ci.nbrOffset + size_t(r) * ci.nbrPitch + jj : 0u];
+            partner[s][r] = (live[s] && row) ? id : kNoNeighbor;
         }
-        if (ok) ok = !front_occupied<LAT>(d.lattice, mv);
-        uint8_t verdict = 0;
-        if (ok) {
-            atomicOr(&T[mv.to >> 5], 1u << (mv.to & 31u));
-            verdict = uint8_t(1u + dir);
-        }
-        d.flags[slot] = verdict;
     }
+    // Round 2: partner codes and the front plane.  CRITICAL.cu:322-323: the
+    // draw is keyed by the species-local index.
+    uint32_t dir[SLOTS], to[SLOTS], partnerCode[SLOTS][MAXDEG], occ[SLOTS];
+#pragma unroll
+    for (int s = 0; s < SLOTS; ++s) {
+        dir[s] = move_hash(move_hash(j[s]) ^ draw.seed) % 6u;
+        const Move mv = make_move(m[s], dir[s], d.mask);
+        to[s] = mv.to;
+#pragma unroll
+        for (int r = 0; r < MAXDEG; ++r)
+            partnerCode[s][r] = d.pos[partner[s][r] != kNoNeighbor ? partner[s][r] : 0u];
+        occ[s] = plane_any<LAT == kLatticeU8 ? 3 : 6>(static_cast<const uint64_t*>(d.lattice), mv);
+    }
+    bool ok[SLOTS];
+#pragma unroll
+    for (int s = 0; s < SLOTS; ++s) {
+        uint32_t bad = 0;
+#pragma unroll
+        for (int r = 0; r < MAXDEG; ++r) {
+            const uint32_t key = bond_key(partnerCode[s][r], to[s]);
+            if (partner[s][r] != kNoNeighbor) bad |= sForbidden[key >> 5] >> (key & 31u);
+        }
+        ok[s] = live[s] && !(bad & 1u) && occ[s] == 0u;
+    }
+
+    // Mark targets and append survivors to this workgroup's list.
+    uint32_t* T = d.proposal + size_t(t & 1u) * d.latWords;
+    uint2* list = d.list + size_t(lb) * (SLOTS * B);
+    const uint32_t lane = threadIdx.x & 63u;
+#pragma unroll
+    for (int s = 0; s < SLOTS; ++s) {
+        const uint64_t mask = __ballot(ok[s]);
+        uint32_t waveBase = 0;
+        if (lane == 0 && mask != 0) waveBase = atomicAdd(&sCount, uint32_t(__popcll(mask)));
+        waveBase = __shfl(waveBase, 0, 64);
+        if (ok[s]) {
+            atomicOr(&T[to[s] >> 5], 1u << (to[s] & 31u));
+            list[waveBase + lane_rank(mask)] = make_uint2(j[s] | (dir[s] << kDirShift), m[s]);
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) d.listCount[lb] = sCount;
 }
 
 __device__ __forceinline__ uint32_t block_sum(uint32_t v, uint32_t* scratch) {
@@ -147,42 +211,49 @@ __device__ __forceinline__ uint32_t block_sum(uint32_t v, uint32_t* scratch) {
     return total;
 }
 
-template <int LAT>
-__global__ __launch_bounds__(1024) void k_perform(SysDesc d, uint32_t k) {
-    __shared__ uint32_t sSum[16];
+template <int LAT, int SLOTS>
+__global__ __launch_bounds__(256) void k_perform(SysDesc d, uint32_t k) {
+    __shared__ uint32_t sSum[4];
     DevState* st = d.state;
     const uint64_t t = st->substep + k;
     if (t >= st->stopAt) return;
     const SubstepDraw draw = substep_draw(d.seed, t, d.nColors);
     const ColorInfo ci = d.colors[draw.species];
-    const uint32_t lb = xcd_block();
-    const uint32_t nb = gridDim.x;
+    const uint32_t lb = xcd_block();  // same numbering as pass 1: list lb
+    const uint32_t B = blockDim.x;
+    const uint32_t count = d.listCount[lb];
+    const uint2* list = d.list + size_t(lb) * (SLOTS * B);
     const uint32_t* T = d.proposal + size_t(t & 1u) * d.latWords;
 
+    // The list is compacted, so about a fifth of its capacity is filled: walk
+    // only the entries that exist, one per thread and round.
+    const uint64_t* T64 = reinterpret_cast<const uint64_t*>(T);
     uint32_t accepted = 0;
-    for (uint32_t j = lb * blockDim.x + threadIdx.x; j < ci.count; j += nb * blockDim.x) {
-        const uint32_t slot = ci.slotOffset + j;
-        const uint32_t verdict = d.flags[slot];
-        if (verdict == 0u) continue;
-        const uint32_t dir = verdict - 1u;
-        const uint32_t m = d.pos[slot];
+    for (uint32_t idx = threadIdx.x; idx < count; idx += B) {
+        const uint2 e = list[idx];
+        const uint32_t jj = e.x & ((1u << kDirShift) - 1u);
+        const uint32_t dir = e.x >> kDirShift;
+        const uint32_t m = e.y;
         const Move mv = make_move(m, dir, d.mask);
-        if (front_proposed(T, mv)) continue;
+        if (plane_any<6>(T64, mv)) continue;
+        const uint32_t slot = ci.slotOffset + jj;
         d.pos[slot] = mv.to;
         if (LAT == kLatticeU8) {
             uint8_t* lat = static_cast<uint8_t*>(d.lattice);
             lat[m] = 0;
             lat[mv.to] = 1;
         } else {
-            uint32_t* lat = static_cast<uint32_t*>(d.lattice);
-            const uint32_t wFrom = m >> 5, wTo = mv.to >> 5;
-            const uint32_t bFrom = 1u << (m & 31u), bTo = 1u << (mv.to & 31u);
-            if (wFrom == wTo) {
-                atomicXor(&lat[wFrom], bFrom | bTo);
-            } else {
-                atomicXor(&lat[wFrom], bFrom);
-                atomicXor(&lat[wTo], bTo);
-            }
+            // Byte b of the bit lattice holds the aligned 2x2x2 block of
+            // cells 8b..8b+7 (the low three code bits are the lowest bit of
+            // x, y and z).  Two corners in one block would overlap, and an
+            // accepted move's old and new corner cannot share a block with
+            // any other monomer's old or new corner, so every byte touched
+            // here has exactly one writer and its new value is known:
+            // 0 for the vacated block, the single new bit for the entered one.
+            uint8_t* lat = static_cast<uint8_t*>(d.lattice);
+            const uint32_t bFrom = m >> 3, bTo = mv.to >> 3;
+            if (bFrom != bTo) lat[bFrom] = 0;
+            lat[bTo] = uint8_t(1u << (mv.to & 7u));
         }
         if (mv.wraps) {
             const uint32_t axis = dir >> 1;
@@ -193,7 +264,7 @@ __global__ __launch_bounds__(1024) void k_perform(SysDesc d, uint32_t k) {
     }
     const uint32_t total = block_sum(accepted, sSum);
     if (threadIdx.x == 0 && total != 0u) {
-        atomicAdd(reinterpret_cast<unsigned long long*>(&st->accepted), (unsigned long long)total);
+        st->acceptedByBlock[blockIdx.x] += total;  // this workgroup's own entry
         if (st->log != nullptr) {
             const uint64_t idx = t - st->logFirst;
             if (idx < st->logCapacity) atomicAdd(&st->log[idx], total);
@@ -212,22 +283,48 @@ __global__ void k_fill_u32(uint32_t* p, uint64_t n, uint32_t v) {
         p[i] = v;
 }
 
-}  // namespace
-
-hipError_t launch_check(const SysDesc& d, uint32_t k, const LaunchShape& s, hipStream_t st) {
-    if (d.latticeKind == kLatticeU8)
-        hipLaunchKernelGGL(k_check<kLatticeU8>, dim3(s.checkGrid), dim3(s.block), 0, st, d, k);
+template <int LAT, int SLOTS>
+hipError_t check_by_degree(const SysDesc& d, uint32_t k, const LaunchShape& s, hipStream_t st) {
+    const dim3 g(s.grid), b(s.block);
+    if (d.maxDegree <= 2)
+        hipLaunchKernelGGL((k_check<LAT, SLOTS, 2>), g, b, 0, st, d, k);
+    else if (d.maxDegree <= 4)
+        hipLaunchKernelGGL((k_check<LAT, SLOTS, 4>), g, b, 0, st, d, k);
     else
-        hipLaunchKernelGGL(k_check<kLatticeBit>, dim3(s.checkGrid), dim3(s.block), 0, st, d, k);
+        hipLaunchKernelGGL((k_check<LAT, SLOTS, 8>), g, b, 0, st, d, k);
     return hipGetLastError();
 }
 
-hipError_t launch_perform(const SysDesc& d, uint32_t k, const LaunchShape& s, hipStream_t st) {
-    if (d.latticeKind == kLatticeU8)
-        hipLaunchKernelGGL(k_perform<kLatticeU8>, dim3(s.performGrid), dim3(s.block), 0, st, d, k);
-    else
-        hipLaunchKernelGGL(k_perform<kLatticeBit>, dim3(s.performGrid), dim3(s.block), 0, st, d, k);
+template <int LAT>
+hipError_t check_by_slots(const SysDesc& d, uint32_t k, const LaunchShape& s, hipStream_t st) {
+    switch (s.slots) {
+        case 1: return check_by_degree<LAT, 1>(d, k, s, st);
+        case 2: return check_by_degree<LAT, 2>(d, k, s, st);
+        default: return check_by_degree<LAT, 4>(d, k, s, st);
+    }
+}
+
+template <int LAT>
+hipError_t perform_by_slots(const SysDesc& d, uint32_t k, const LaunchShape& s, hipStream_t st) {
+    const dim3 g(s.grid), b(s.block);
+    switch (s.slots) {
+        case 1: hipLaunchKernelGGL((k_perform<LAT, 1>), g, b, 0, st, d, k); break;
+        case 2: hipLaunchKernelGGL((k_perform<LAT, 2>), g, b, 0, st, d, k); break;
+        default: hipLaunchKernelGGL((k_perform<LAT, 4>), g, b, 0, st, d, k); break;
+    }
     return hipGetLastError();
+}
+
+}  // namespace
+
+hipError_t launch_check(const SysDesc& d, uint32_t k, const LaunchShape& s, hipStream_t st) {
+    return d.latticeKind == kLatticeU8 ? check_by_slots<kLatticeU8>(d, k, s, st)
+                                       : check_by_slots<kLatticeBit>(d, k, s, st);
+}
+
+hipError_t launch_perform(const SysDesc& d, uint32_t k, const LaunchShape& s, hipStream_t st) {
+    return d.latticeKind == kLatticeU8 ? perform_by_slots<kLatticeU8>(d, k, s, st)
+                                       : perform_by_slots<kLatticeBit>(d, k, s, st);
 }
 
 hipError_t launch_advance(DevState* state, uint32_t count, hipStream_t st) {

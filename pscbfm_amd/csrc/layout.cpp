@@ -49,6 +49,10 @@ SystemLayout build_layout(const SystemInput& in, const std::vector<uint8_t>* col
         ++count[L.color[i]];
         maxDeg[L.color[i]] = std::max(maxDeg[L.color[i]], in.bonds.degree(i));
     }
+    for (uint32_t c = 0; c < nColors; ++c)
+        if (count[c] == 0)
+            throw std::invalid_argument("species " + std::to_string(c) +
+                                        " has no monomers; colors must be numbered 0..k-1 without gaps");
 
     L.colors.resize(nColors);
     uint32_t slot = 0, nbrOff = 0;

@@ -207,21 +207,24 @@ void Updater::setColors(const uint8_t* colors, uint32_t n) {
     hasColorOverride_ = true;
 }
 
-void Updater::setLaunchShape(uint32_t block, uint32_t checkCap, uint32_t performCap) {
-    if (block != 0 && (block % 64 != 0 || block > 1024))
-        throw std::invalid_argument("[setLaunchShape] block must be a multiple of 64 up to 1024");
+void Updater::setLaunchShape(uint32_t block, uint32_t slots) {
+    if (block != 0 && (block % 64 != 0 || block > 256))
+        throw std::invalid_argument("[setLaunchShape] block must be 64, 128, 192 or 256");
+    if (slots != 0 && slots != 1 && slots != 2 && slots != 4)
+        throw std::invalid_argument("[setLaunchShape] slots per thread must be 1, 2 or 4");
     blockOverride_ = block;
-    checkCapOverride_ = checkCap;
-    performCapOverride_ = performCap;
-    if (initialized_) {
-        const uint32_t b = block ? block : 256;
-        shape_.block = b;
-        shape_.checkGrid = grid_for(layout_.maxColorCount, b, checkCap ? checkCap : (1u << 20));
-        shape_.performGrid = grid_for(layout_.maxColorCount, b, performCap ? performCap : 4096u);
-        if (graphExec_) {
-            (void)hipGraphExecDestroy(graphExec_);
-            graphExec_ = nullptr;
-        }
+    slotsOverride_ = slots;
+    if (initialized_) applyLaunchShape();
+}
+
+void Updater::applyLaunchShape() {
+    shape_.block = blockOverride_ ? blockOverride_ : 256;
+    shape_.slots = slotsOverride_ ? slotsOverride_ : 2;
+    shape_.grid = grid_for(layout_.maxColorCount, shape_.block * shape_.slots);
+    performGridMax_ = std::max(performGridMax_, shape_.grid);
+    if (graphExec_) {
+        (void)hipGraphExecDestroy(graphExec_);
+        graphExec_ = nullptr;
     }
 }
 
@@ -336,19 +339,23 @@ void Updater::initialize() {
         uint64_t bytes;
         uint64_t off;
     };
-    enum { P_POS, P_IMGX, P_IMGY, P_IMGZ, P_FLAGS, P_NBR, P_LAT, P_PROP, P_COLORS, P_BOND, P_STATE, P_N };
+    enum { P_POS, P_IMGX, P_IMGY, P_IMGZ, P_LIST, P_LCOUNT, P_NBR, P_LAT, P_PROP, P_COLORS, P_BOND, P_STATE,
+           P_ACC, P_N };
+    const uint64_t gridCap = max_grid(layout_.maxColorCount);
     Piece p[P_N];
     p[P_POS].bytes = nSlots * 4;
     p[P_IMGX].bytes = nSlots * 4;
     p[P_IMGY].bytes = nSlots * 4;
     p[P_IMGZ].bytes = nSlots * 4;
-    p[P_FLAGS].bytes = nSlots;
+    p[P_LIST].bytes = list_capacity(layout_.maxColorCount) * 8;
+    p[P_LCOUNT].bytes = gridCap * 4;
     p[P_NBR].bytes = std::max<uint64_t>(layout_.nbr.size(), 1) * 4;
     p[P_LAT].bytes = latticeKind_ == kLatticeU8 ? cells_ : latWords * 4;
     p[P_PROP].bytes = 2 * latWords * 4;
     p[P_COLORS].bytes = std::max<uint64_t>(layout_.colors.size(), 1) * sizeof(ColorInfo);
     p[P_BOND].bytes = 64;
     p[P_STATE].bytes = sizeof(DevState);
+    p[P_ACC].bytes = gridCap * sizeof(uint64_t);
     uint64_t total = 0;
     for (int i = 0; i < P_N; ++i) {
         p[i].off = total;
@@ -363,7 +370,8 @@ void Updater::initialize() {
     desc_.imgX = reinterpret_cast<int32_t*>(base + p[P_IMGX].off);
     desc_.imgY = reinterpret_cast<int32_t*>(base + p[P_IMGY].off);
     desc_.imgZ = reinterpret_cast<int32_t*>(base + p[P_IMGZ].off);
-    desc_.flags = reinterpret_cast<uint8_t*>(base + p[P_FLAGS].off);
+    desc_.list = reinterpret_cast<uint2*>(base + p[P_LIST].off);
+    desc_.listCount = reinterpret_cast<uint32_t*>(base + p[P_LCOUNT].off);
     desc_.nbr = reinterpret_cast<const uint32_t*>(base + p[P_NBR].off);
     desc_.lattice = base + p[P_LAT].off;
     desc_.proposal = reinterpret_cast<uint32_t*>(base + p[P_PROP].off);
@@ -374,6 +382,8 @@ void Updater::initialize() {
     for (int a = 0; a < 3; ++a) desc_.mask[a] = layout_.codec.mask[a];
     desc_.nColors = uint32_t(layout_.colors.size());
     desc_.latWords = uint32_t(latWords);
+    desc_.maxDegree = 0;
+    for (const ColorInfo& ci : layout_.colors) desc_.maxDegree = std::max(desc_.maxDegree, ci.maxDegree);
     desc_.latticeKind = latticeKind_;
     desc_.maxColorCount = layout_.maxColorCount;
 
@@ -407,16 +417,14 @@ void Updater::initialize() {
     }
     DevState st{};
     st.substep = 0;
-    st.accepted = 0;
     st.log = nullptr;
     st.stopAt = std::numeric_limits<uint64_t>::max();
+    st.acceptedByBlock = reinterpret_cast<uint64_t*>(base + p[P_ACC].off);
     up(P_STATE, &st, sizeof st);
     hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize (initialize)");
 
-    const uint32_t b = blockOverride_ ? blockOverride_ : 256;
-    shape_.block = b;
-    shape_.checkGrid = grid_for(layout_.maxColorCount, b, checkCapOverride_ ? checkCapOverride_ : (1u << 20));
-    shape_.performGrid = grid_for(layout_.maxColorCount, b, performCapOverride_ ? performCapOverride_ : 4096u);
+    performGridMax_ = 0;
+    applyLaunchShape();
     substep_ = 0;
     acceptedTotal_ = 0;
     initialized_ = true;
@@ -557,8 +565,14 @@ void Updater::run(int32_t nMCS, uint32_t* log, bool profile, bool useGraph) {
     {
         DevState st{};
         hip_check(hipMemcpy(&st, desc_.state, sizeof st, hipMemcpyDeviceToHost), "state D2H");
-        lastRun_.accepted = st.accepted - acceptedTotal_;
-        acceptedTotal_ = st.accepted;
+        std::vector<uint64_t> perBlock(performGridMax_);
+        hip_check(hipMemcpy(perBlock.data(), st.acceptedByBlock, perBlock.size() * sizeof(uint64_t),
+                            hipMemcpyDeviceToHost),
+                  "accepted D2H");
+        uint64_t total = 0;
+        for (uint64_t v : perBlock) total += v;
+        lastRun_.accepted = total - acceptedTotal_;
+        acceptedTotal_ = total;
         if (st.substep != substep_) throw std::runtime_error("device substep counter out of step with the host");
     }
 }

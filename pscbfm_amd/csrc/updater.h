@@ -60,7 +60,8 @@ public:
     void setUniformColoring(bool on) { uniformColors_ = on; }
     void setCheckOnInitialize(bool on) { checkOnInit_ = on; }
     void setColors(const uint8_t* colors, uint32_t n);
-    void setLaunchShape(uint32_t block, uint32_t checkCap, uint32_t performCap);
+    // Threads per workgroup (0 = 256) and monomers per pass-1 thread (0 = 2).
+    void setLaunchShape(uint32_t block, uint32_t slots);
     void setGraphChunk(uint32_t substeps) { graphChunk_ = substeps; }
     void setMonomerCoordinatesBulk(const int32_t* xyz, uint32_t n);
     void setConnectivityBulk(const uint32_t* pairs, uint64_t nPairs);
@@ -97,6 +98,7 @@ private:
     void freeDevice();
     void captureGraph();
     void syncFromDevice();
+    void applyLaunchShape();
 
     // staging
     int gpu_ = -1;
@@ -116,7 +118,7 @@ private:
     bool uniformColors_ = true;
     bool checkOnInit_ = true;
     uint32_t graphChunk_ = 0;   // 0: choose automatically
-    uint32_t blockOverride_ = 0, checkCapOverride_ = 0, performCapOverride_ = 0;
+    uint32_t blockOverride_ = 0, slotsOverride_ = 0;
 
     // device
     bool initialized_ = false;
@@ -128,6 +130,7 @@ private:
     LaunchShape shape_;
     hipGraphExec_t graphExec_ = nullptr;
     uint32_t graphSubsteps_ = 0;
+    uint32_t performGridMax_ = 0;  // largest pass-2 grid used (entries of acceptedByBlock)
     uint64_t substep_ = 0;
     uint64_t acceptedTotal_ = 0;
     uint64_t cells_ = 0;

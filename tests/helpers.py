@@ -61,7 +61,7 @@ def forbidden_table() -> np.ndarray:
 
 
 def make_updater(s: Sys, seed: int, kind: int = P.LATTICE_BIT, colors=None, gpu: int = 0, bulk: bool = True,
-                 block: int = 0) -> P.UpdaterGPUScBFM_AB_Type:
+                 block: int = 0, slots: int = 0) -> P.UpdaterGPUScBFM_AB_Type:
     """Configures an updater in the call order of GPUScBFM_AB_Type.h:83-130."""
     u = P.UpdaterGPUScBFM_AB_Type()
     u.setGpu(gpu)
@@ -86,8 +86,8 @@ def make_updater(s: Sys, seed: int, kind: int = P.LATTICE_BIT, colors=None, gpu:
     u.setLatticeKind(kind)
     if colors is not None:
         u.setColors(colors)
-    if block:
-        u.setLaunchShape(block, 0, 0)
+    if block or slots:
+        u.setLaunchShape(block, slots)
     u.initialize()
     return u
 
@@ -98,10 +98,10 @@ def oracle_for(s: Sys, colors, seed: int) -> oracle.System:
 
 
 def gpu_vs_oracle(s: Sys, seed: int, n_mcs: int, kind: int = P.LATTICE_BIT, chunks=(None,), use_graph=True,
-                  block: int = 0):
+                  block: int = 0, slots: int = 0):
     """Runs n_mcs sweeps on the GPU (split into `chunks` calls) and on the
     oracle; returns (gpu_xyz, gpu_log, oracle_xyz, oracle_log)."""
-    u = make_updater(s, seed, kind, block=block)
+    u = make_updater(s, seed, kind, block=block, slots=slots)
     colors = u.colors(s.n)
     logs = []
     plan = [n_mcs] if chunks == (None,) else list(chunks)

@@ -58,10 +58,10 @@ def test_star_network_multicolor():
     assert np.array_equal(gx, ox)
 
 
-@pytest.mark.parametrize("block", [64, 512, 1024])
-def test_launch_shape_does_not_change_results(block):
+@pytest.mark.parametrize("block,slots", [(64, 1), (64, 4), (128, 2), (256, 1), (192, 4)])
+def test_launch_shape_does_not_change_results(block, slots):
     s = chains((64, 64, 64), 30, 40, seed=2)
-    gx, glog, ox, olog = gpu_vs_oracle(s, seed=8, n_mcs=10, block=block)
+    gx, glog, ox, olog = gpu_vs_oracle(s, seed=8, n_mcs=10, block=block, slots=slots)
     assert np.array_equal(glog, olog)
     assert np.array_equal(gx, ox)
 
