@@ -196,10 +196,20 @@ def main() -> int:
     log(f"pass 1 mean {k1_mean_ms * 1e3:.1f} us ({k1_gbs:.0f} GB/s algorithmic), "
         f"pass 2 mean {k2_mean_ms * 1e3:.1f} us")
 
+    # Plain streaming copy of 512 MiB: achievable-bandwidth reference, and the
+    # kernel whose byte counters scale the profiler's traffic figures.
+    copy_bytes = 1 << 29
+    copy_ms = P.copy_probe(0, copy_bytes, 10)
+    copy_gbs = 2 * copy_bytes / (copy_ms * 1e-3) / 1e9
+    log(f"copy reference: {copy_gbs:.0f} GB/s (read+write)")
+
+    # HBM bytes per pass-1 launch, from the committed rocprofv3 FETCH_SIZE /
+    # WRITE_SIZE runs of this workload (profiles/), when present.
     traffic = None
-    tfile = os.environ.get("PSCBFM_TRAFFIC_BYTES")
-    if tfile:
-        traffic = float(tfile)
+    tfile = ROOT / "profiles" / f"dense_melt_{L}_{args.lattice}.json"
+    if tfile.exists() and args.chain == 256 and abs(args.phi - 0.5) < 1e-9:
+        tr = json.loads(tfile.read_text()).get("traffic") or {}
+        traffic = tr.get("kernels", {}).get("k_check", {}).get("traffic_bytes")
 
     out = {
         "metric": METRIC,
@@ -238,6 +248,7 @@ def main() -> int:
         "sweep_algorithmic_GBps": (p1 + p2) / gpu_s / 1e9,
         "sweep_roofline_frac": (p1 + p2) / gpu_s / 1e9 / HBM_PEAK_GBS,
         "pass2_mean_us": k2_mean_ms * 1e3,
+        "copy_reference_GBps": copy_gbs,
         "wall_ms_incl_copyback": wall * 1e3,
     }
     if not args.no_cpu:

@@ -175,6 +175,11 @@ int pscbfm_gen_star_network(uint32_t lx, uint32_t ly, uint32_t lz, uint32_t n_st
                             uint32_t arm_length, uint64_t seed, int32_t* xyz, uint32_t* bonds,
                             uint64_t bond_capacity, uint64_t* n_bonds);
 
+/* Streaming-copy reference on `gpu`: copies `bytes` (read + write) `repeats`
+ * times with a plain 16-byte-per-lane kernel and returns the mean time.  Used
+ * to scale profiler byte counters and as the achievable-bandwidth reference. */
+int pscbfm_copy_probe(int gpu, uint64_t bytes, int repeats, double* ms_per_copy);
+
 /* Library version string. */
 const char* pscbfm_version(void);
 

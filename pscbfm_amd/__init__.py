@@ -109,6 +109,7 @@ _SIGNATURES = {
     "pscbfm_gen_linear_chains": (C.c_int, [C.c_uint32] * 5 + [C.c_uint64, C.c_uint32, _i32p]),
     "pscbfm_gen_linear_chains_ordered": (C.c_int, [C.c_uint32] * 5 + [_i32p]),
     "pscbfm_gen_star_network": (C.c_int, [C.c_uint32] * 6 + [C.c_uint64, _i32p, _u32p, C.c_uint64, _u64p]),
+    "pscbfm_copy_probe": (C.c_int, [C.c_int, C.c_uint64, C.c_int, C.POINTER(C.c_double)]),
     "pscbfm_version": (C.c_char_p, []),
 }
 
@@ -350,6 +351,13 @@ def color_graph(n: int, start: np.ndarray, ids: np.ndarray, uniform: bool = True
     _raise(lib().pscbfm_color_graph(int(n), _ptr(s, C.c_uint32), _ptr(i, C.c_uint32), int(bool(uniform)),
                                     _ptr(out, C.c_uint8)), "graph coloring")
     return out[:n]
+
+
+def copy_probe(gpu: int = 0, nbytes: int = 1 << 29, repeats: int = 10) -> float:
+    """Mean time (ms) of a plain device copy of nbytes (read nbytes + write nbytes)."""
+    ms = C.c_double()
+    _raise(lib().pscbfm_copy_probe(int(gpu), int(nbytes), int(repeats), C.byref(ms)), "copy probe failed")
+    return ms.value
 
 
 def chain_bonds(n_chains: int, length: int) -> np.ndarray:

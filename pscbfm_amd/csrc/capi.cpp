@@ -292,6 +292,8 @@ int pscbfm_count_proposal_bits(const pscbfm_updater* h, uint64_t* bits_even, uin
 int pscbfm_zcurve_encode(uint32_t lx, uint32_t ly, uint32_t lz, uint32_t x, uint32_t y, uint32_t z,
                          uint32_t* code) {
     if (!code) return PSCBFM_INVALID_ARGUMENT;
+    auto p2 = [](uint32_t v) { return v >= 8 && (v & (v - 1)) == 0; };
+    if (!p2(lx) || !p2(ly) || !p2(lz) || uint64_t(lx) * ly * lz > (uint64_t(1) << 32)) return PSCBFM_INVALID_ARGUMENT;
     pscbfm::MortonCodec c;
     c.init(lx, ly, lz);
     *code = c.encode(x & (lx - 1), y & (ly - 1), z & (lz - 1));
@@ -383,6 +385,36 @@ int pscbfm_gen_star_network(uint32_t lx, uint32_t ly, uint32_t lz, uint32_t n_st
     } catch (...) {
         return PSCBFM_INTERNAL_ERROR;
     }
+}
+
+int pscbfm_copy_probe(int gpu, uint64_t bytes, int repeats, double* ms_per_copy) {
+    if (bytes < 16 || repeats < 1 || !ms_per_copy) return PSCBFM_INVALID_ARGUMENT;
+    void* src = nullptr;
+    void* dst = nullptr;
+    hipStream_t st = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = PSCBFM_OK;
+    const uint64_t n16 = bytes / 16;
+    auto ok = [&](hipError_t e) {
+        if (e != hipSuccess && rc == PSCBFM_OK) rc = PSCBFM_DEVICE_ERROR;
+        return rc == PSCBFM_OK;
+    };
+    if (ok(hipSetDevice(gpu)) && ok(hipStreamCreateWithFlags(&st, hipStreamNonBlocking)) &&
+        ok(hipMalloc(&src, n16 * 16)) && ok(hipMalloc(&dst, n16 * 16)) && ok(hipEventCreate(&e0)) &&
+        ok(hipEventCreate(&e1)) && ok(pscbfm::launch_fill_u32(static_cast<uint32_t*>(src), n16 * 4, 1u, st)) &&
+        ok(pscbfm::launch_copy_u4(src, dst, n16, st)) && ok(hipEventRecord(e0, st))) {
+        for (int r = 0; r < repeats && ok(pscbfm::launch_copy_u4(src, dst, n16, st)); ++r) {
+        }
+        float ms = 0;
+        if (ok(hipEventRecord(e1, st)) && ok(hipEventSynchronize(e1)) && ok(hipEventElapsedTime(&ms, e0, e1)))
+            *ms_per_copy = double(ms) / repeats;
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (src) (void)hipFree(src);
+    if (dst) (void)hipFree(dst);
+    if (st) (void)hipStreamDestroy(st);
+    return rc;
 }
 
 const char* pscbfm_version(void) { return "pscbfm-mi355x 0.1 (gfx950)"; }

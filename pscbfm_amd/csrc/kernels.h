@@ -25,6 +25,8 @@ hipError_t launch_perform(const SysDesc& d, uint32_t k, const LaunchShape& s, hi
 hipError_t launch_advance(DevState* state, uint32_t count, hipStream_t st);
 // Fills n 32-bit words with v (graph-safe replacement of a memset).
 hipError_t launch_fill_u32(uint32_t* p, uint64_t n, uint32_t v, hipStream_t st);
+// Streaming copy of n16 16-byte elements (bandwidth and counter reference).
+hipError_t launch_copy_u4(const void* src, void* dst, uint64_t n16, hipStream_t st);
 
 // Workgroups that cover `items` with `perGroup` items each, rounded up to
 // whole XCD rounds (multiples of 8).

@@ -1,0 +1,125 @@
+"""Summaries of rocprofv3 CSV output for the sweep kernels.
+
+    python -m pscbfm_amd.profiling --stats DIR [--fetch DIR --write DIR]
+        [--copy-bytes N] [--md OUT.md] [--json OUT.json] [--command "..."]
+
+--stats   directory of a `rocprofv3 --kernel-trace --stats --output-format csv` run
+--fetch   directory of a `rocprofv3 --pmc FETCH_SIZE --kernel-trace --output-format csv` run
+--write   directory of a `rocprofv3 --pmc WRITE_SIZE --kernel-trace --output-format csv` run
+
+HBM traffic per launch = FETCH_SIZE and WRITE_SIZE (KiB per dispatch) of a
+kernel, each scaled by the factor that makes the same counter of the plain
+copy kernel (k_copy_u4, which reads and writes exactly --copy-bytes) read
+--copy-bytes.
+"""
+from __future__ import annotations
+
+import argparse
+import csv
+import json
+from collections import defaultdict
+from pathlib import Path
+from typing import Dict, Optional
+
+KERNELS = {"k_check": "pass 1 (k_check)", "k_perform": "pass 2 (k_perform)", "k_copy_u4": "copy reference",
+           "k_advance": "counter advance"}
+
+
+def _short(name: str) -> str:
+    for k in KERNELS:
+        if k in name:
+            return k
+    return name.split("(")[0][-40:]
+
+
+def _find(d: Path, suffix: str) -> Path:
+    hits = sorted(Path(d).rglob(f"*{suffix}"))
+    if not hits:
+        raise FileNotFoundError(f"no *{suffix} under {d}")
+    return hits[0]
+
+
+def _tail(v: list, last: Optional[int]) -> list:
+    return v[-last:] if last else v
+
+
+def kernel_durations(stats_dir: Path, last: Optional[int] = None) -> Dict[str, dict]:
+    """Per kernel: calls, total and mean duration (us) from the kernel trace
+    (mean over the last `last` dispatches of each kernel when given)."""
+    rows = sorted(csv.DictReader(open(_find(stats_dir, "kernel_trace.csv"))), key=lambda r: int(r["Start_Timestamp"]))
+    acc: Dict[str, list] = defaultdict(list)
+    for r in rows:
+        acc[_short(r["Kernel_Name"])].append((int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3)
+    out = {}
+    for k, v in acc.items():
+        t = _tail(v, last if k not in ("k_copy_u4",) else None)
+        out[k] = {"calls": len(v), "total_us": sum(v), "mean_us": sum(t) / len(t), "mean_over": len(t)}
+    return out
+
+
+def counter_means(pmc_dir: Path, counter: str, last: Optional[int] = None) -> Dict[str, float]:
+    rows = sorted(csv.DictReader(open(_find(pmc_dir, "counter_collection.csv"))), key=lambda r: int(r["Dispatch_Id"]))
+    acc: Dict[str, list] = defaultdict(list)
+    for r in rows:
+        if r["Counter_Name"] == counter:
+            acc[_short(r["Kernel_Name"])].append(float(r["Counter_Value"]))
+    return {k: (lambda t: sum(t) / len(t))(_tail(v, last if k != "k_copy_u4" else None)) for k, v in acc.items()}
+
+
+def traffic(fetch_dir: Path, write_dir: Path, copy_bytes: int, last: Optional[int] = None) -> dict:
+    fetch = counter_means(fetch_dir, "FETCH_SIZE", last)
+    write = counter_means(write_dir, "WRITE_SIZE", last)
+    kib = copy_bytes / 1024.0
+    f_r = kib / fetch["k_copy_u4"]
+    f_w = kib / write["k_copy_u4"]
+    out = {"copy_bytes": copy_bytes, "fetch_scale": f_r, "write_scale": f_w, "kernels": {}}
+    for k in fetch:
+        if k in write:
+            r = fetch[k] * f_r * 1024.0
+            w = write[k] * f_w * 1024.0
+            out["kernels"][k] = {"fetch_kib": fetch[k], "write_kib": write[k], "read_bytes": r, "write_bytes": w,
+                                 "traffic_bytes": r + w}
+    return out
+
+
+def main(argv: Optional[list] = None) -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--stats", type=Path, required=True)
+    ap.add_argument("--fetch", type=Path)
+    ap.add_argument("--write", type=Path)
+    ap.add_argument("--copy-bytes", type=int, default=1 << 29)
+    ap.add_argument("--md", type=Path)
+    ap.add_argument("--json", type=Path)
+    ap.add_argument("--command", default="")
+    ap.add_argument("--title", default="rocprofv3 summary")
+    ap.add_argument("--last", type=int, default=None,
+                    help="average only the last N dispatches of each sweep kernel (steady state)")
+    args = ap.parse_args(argv)
+    dur = kernel_durations(args.stats, args.last)
+    tr = traffic(args.fetch, args.write, args.copy_bytes, args.last) if args.fetch and args.write else None
+    lines = [f"# {args.title}", ""]
+    if args.command:
+        lines += ["Commands (run from the repository root on one MI355X):", "", "```", args.command, "```", ""]
+    lines += ["| kernel | calls | mean duration (us) | over last | total (us) |", "|---|---|---|---|---|"]
+    for k, v in sorted(dur.items(), key=lambda kv: -kv[1]["total_us"]):
+        lines.append(f"| {KERNELS.get(k, k)} | {v['calls']} | {v['mean_us']:.2f} | {v['mean_over']} | "
+                     f"{v['total_us']:.1f} |")
+    if tr:
+        lines += ["", f"HBM traffic per launch (copy reference {tr['copy_bytes']} B: FETCH_SIZE scale "
+                      f"{tr['fetch_scale']:.3f}, WRITE_SIZE scale {tr['write_scale']:.3f}):", "",
+                  "| kernel | FETCH_SIZE KiB | WRITE_SIZE KiB | read MB | write MB | total MB |", "|---|---|---|---|---|---|"]
+        for k, v in tr["kernels"].items():
+            lines.append(f"| {KERNELS.get(k, k)} | {v['fetch_kib']:.0f} | {v['write_kib']:.0f} | "
+                         f"{v['read_bytes'] / 1e6:.1f} | {v['write_bytes'] / 1e6:.1f} | {v['traffic_bytes'] / 1e6:.1f} |")
+    text = "\n".join(lines) + "\n"
+    if args.md:
+        args.md.write_text(text)
+    else:
+        print(text)
+    if args.json:
+        args.json.write_text(json.dumps({"durations": dur, "traffic": tr}, indent=1) + "\n")
+    return 0
+
+
+if __name__ == "__main__":
+    raise SystemExit(main())

@@ -1,0 +1,326 @@
+"""CPU tests of the library boundary and host logic (no GPU work).
+
+Covers: every symbol of include/pscbfm.h is exported; the reference's error
+behaviour (UpdaterGPUScBFM_AB_Type.cu: 472-487, 521-528, 699-726, 728-753,
+777-795, 797-848, 882-986) through the Python mirror and the C++ drop-in
+header; the graph coloring against the reference's own graphColoring.tpp
+(compiled from the reference tree into oracle/_ref); BFM file I/O on the
+reference's fixtures; the synthetic system builders.
+"""
+import ctypes as C
+import itertools
+import re
+import shutil
+import subprocess
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import oracle
+import pscbfm_amd as P
+from pscbfm_amd.bfm import read_bfm, write_bfm
+from helpers import GOLDEN, Sys, chains, oracle_for, ordered_chains, stars
+
+ROOT = Path(__file__).resolve().parents[1]
+
+
+# ---- exports ----------------------------------------------------------------------
+def _declared(header: Path):
+    text = header.read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(pscbfm_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_every_declared_symbol_is_exported():
+    names = _declared(ROOT / "include" / "pscbfm.h")
+    assert len(names) > 40
+    lib = C.CDLL(str(P.library_path))
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_library_is_a_gfx950_code_object():
+    data = P.library_path.read_bytes()
+    assert b"gfx950" in data
+
+
+# ---- error behaviour of the reference surface ------------------------------------
+def _filled(n=2, box=(16, 16, 16)):
+    u = P.UpdaterGPUScBFM_AB_Type()
+    u.setPeriodicity(True, True, True)
+    u.setNrOfAllMonomers(n)
+    return u
+
+
+def test_set_gpu_out_of_range_is_invalid_argument():
+    u = P.UpdaterGPUScBFM_AB_Type()
+    with pytest.raises(P.InvalidArgument, match="not present"):
+        u.setGpu(64)
+    with pytest.raises(P.InvalidArgument):
+        u.setGpu(-1)
+
+
+def test_non_periodic_box_is_invalid_argument():
+    u = P.UpdaterGPUScBFM_AB_Type()
+    for flags in [(False, True, True), (True, False, True), (True, True, False)]:
+        with pytest.raises(P.InvalidArgument, match="periodic"):
+            u.setPeriodicity(*flags)
+
+
+def test_set_nr_of_all_monomers_twice_is_runtime_error():
+    u = _filled()
+    with pytest.raises(RuntimeError, match="already set"):
+        u.setNrOfAllMonomers(5)
+
+
+def test_coordinate_index_is_range_checked():
+    u = _filled(3)
+    with pytest.raises(IndexError):
+        u.setMonomerCoordinates(3, 0, 0, 0)
+    with pytest.raises(IndexError):
+        u.getMonomerPositionInX(7)
+
+
+def test_connectivity_limit_is_invalid_argument():
+    u = _filled(10)
+    for k in range(1, 9):
+        u.setConnectivity(0, k)
+    with pytest.raises(P.InvalidArgument, match="maximum amount of bonds"):
+        u.setConnectivity(0, 9)
+
+
+def test_lattice_size_checks():
+    u = P.UpdaterGPUScBFM_AB_Type()
+    with pytest.raises(RuntimeError, match="power of 2"):
+        u.setLatticeSize(16, 24, 16)
+    with pytest.raises(RuntimeError, match="power of 2"):
+        u.setLatticeSize(16, 16, 48)   # the reference does not check z; here it is checked
+    with pytest.raises(P.InvalidArgument, match="larger than"):
+        u.setLatticeSize(65536, 16, 16)
+    with pytest.raises(RuntimeError, match="at least 8"):
+        u.setLatticeSize(4, 16, 16)
+    u.setLatticeSize(32, 64, 16)
+    u.setLatticeSize(32, 64, 16)   # unchanged: no-op
+
+
+def test_initialize_requires_108_bonds():
+    u = _filled(1)
+    u.setLatticeSize(16, 16, 16)
+    # no copyBondSet calls: every entry allowed -> 512 allowed bonds
+    with pytest.raises(RuntimeError, match="Expected 108 allowed bonds, but got 512"):
+        u.initialize()
+
+
+def test_check_system_detects_overlap_and_forbidden_bond():
+    forb = oracle.standard_bondset()
+
+    def build(xyz, bonds):
+        u = P.UpdaterGPUScBFM_AB_Type()
+        u.setNrOfAllMonomers(len(xyz))
+        for i, p in enumerate(xyz):
+            u.setMonomerCoordinates(i, *p)
+        for a, b in bonds:
+            u.setConnectivity(a, b)
+            u.setConnectivity(b, a)
+        u.setLatticeSize(16, 16, 16)
+        for dx, dy, dz in itertools.product(range(-4, 4), repeat=3):
+            u.copyBondSet(dx, dy, dz, bool(forb[oracle.bond_index(dx, dy, dz)]))
+        return u
+
+    build([(0, 0, 0), (2, 0, 0)], [(0, 1)]).checkSystem()
+    build([(15, 0, 0), (1, 0, 0)], [(0, 1)]).checkSystem()   # bond across the boundary (minimum image)
+    with pytest.raises(RuntimeError, match="excluded volume"):
+        build([(0, 0, 0), (1, 1, 0)], []).checkSystem()
+    with pytest.raises(RuntimeError, match="excluded volume"):
+        build([(0, 0, 0), (15, 15, 15)], []).checkSystem()   # overlap through the periodic boundary
+    with pytest.raises(RuntimeError, match="forbidden bond"):
+        build([(0, 0, 0), (2, 2, 0)], [(0, 1)]).checkSystem()
+
+
+def test_colors_override_must_be_proper_and_gapless():
+    u = _filled(2)
+    u.setConnectivity(0, 1)
+    u.setColors([0, 0])   # stored; validated at initialize (needs a GPU) -- only the count is checked here
+    with pytest.raises(P.InvalidArgument):
+        u.setColors([0])
+
+
+def test_run_before_initialize_is_runtime_error():
+    u = _filled(1)
+    with pytest.raises(RuntimeError, match="initialize"):
+        u.runSimulationOnGPU(1)
+
+
+def test_cleanup_allows_replaying_the_setter_sequence():
+    u = _filled(2)
+    u.cleanup()
+    u.setNrOfAllMonomers(4)   # the reference forgets to reset the count here
+
+
+# ---- C++ drop-in header -----------------------------------------------------------
+CPP_DRIVER = r"""
+#include <cstdio>
+#include <stdexcept>
+#include "UpdaterGPUScBFM_AB_Type.h"
+int main() {
+    int fails = 0;
+    UpdaterGPUScBFM_AB_Type u;
+    try { u.setPeriodicity(true, false, true); fails++; } catch (const std::invalid_argument&) {}
+    u.setNrOfAllMonomers(2);
+    try { u.setNrOfAllMonomers(2); fails++; } catch (const std::runtime_error&) {}
+    try { u.setMonomerCoordinates(5, 0, 0, 0); fails++; } catch (const std::out_of_range&) {}
+    try { u.setLatticeSize(12, 16, 16); fails++; } catch (const std::runtime_error&) {}
+    try { u.setGpu(99); fails++; } catch (const std::invalid_argument&) {}
+    u.setMonomerCoordinates(0, 0, 0, 0);
+    u.setMonomerCoordinates(1, 2, 0, 0);
+    u.setConnectivity(0, 1);
+    u.setConnectivity(1, 0);
+    u.setLatticeSize(16, 16, 16);
+    u.populateLattice();
+    if (u.getMonomerPositionInX(1) != 2) fails++;
+    u.cleanup();
+    std::printf("fails=%d\n", fails);
+    return fails;
+}
+"""
+
+
+@pytest.mark.skipif(shutil.which("g++") is None, reason="no host C++ compiler")
+def test_cpp_dropin_header_compiles_links_and_throws(tmp_path):
+    src = tmp_path / "driver.cpp"
+    src.write_text(CPP_DRIVER)
+    exe = tmp_path / "driver"
+    libdir = P.library_path.parent
+    subprocess.run(["g++", "-std=c++11", "-O1", "-I", str(ROOT / "include"), str(src), "-o", str(exe),
+                    "-L", str(libdir), "-lpscbfm", f"-Wl,-rpath,{libdir}"], check=True)
+    res = subprocess.run([str(exe)], capture_output=True, text=True)
+    assert res.returncode == 0, res.stdout + res.stderr
+    assert "fails=0" in res.stdout
+
+
+# ---- coloring ---------------------------------------------------------------------
+def _csr_from_pairs(n, pairs):
+    s = Sys((8, 8, 8), np.zeros((n, 3), np.int32), np.asarray(pairs, np.uint32).reshape(-1, 2))
+    return s.csr()
+
+
+def _proper(colors, start, ids):
+    for i in range(len(start) - 1):
+        for k in range(start[i], start[i + 1]):
+            if colors[i] == colors[ids[k]]:
+                return False
+    return True
+
+
+def _graphs():
+    rng = np.random.default_rng(3)
+    yield "chain", 50, [(i, i + 1) for i in range(49)]
+    yield "odd ring", 7, [(i, (i + 1) % 7) for i in range(7)]
+    yield "stars", 40, [(0, k) for k in range(1, 5)] + [(k, k + 4) for k in range(1, 30)]
+    yield "isolated", 5, []
+    for trial in range(6):
+        n = 200
+        pairs = set()
+        deg = np.zeros(n, int)
+        for _ in range(300):
+            a, b = (int(v) for v in rng.integers(0, n, 2))
+            if a != b and deg[a] < 8 and deg[b] < 8 and (a, b) not in pairs and (b, a) not in pairs:
+                pairs.add((a, b))
+                deg[a] += 1
+                deg[b] += 1
+        yield f"random{trial}", n, sorted(pairs)
+    for name in ["Ring_N6_n6.bfm", "Hydrogel_NStar_117_NoPerXYZ64.bfm"]:
+        b = read_bfm(GOLDEN / name)
+        yield name, b.n, [tuple(int(v) for v in p) for p in b.bonds]
+
+
+@pytest.mark.parametrize("uniform", [True, False])
+def test_coloring_is_proper(uniform):
+    for name, n, pairs in _graphs():
+        start, ids = _csr_from_pairs(n, pairs)
+        col = P.color_graph(n, start, ids, uniform)
+        assert _proper(col, start, ids), name
+
+
+@pytest.mark.skipif(not oracle.ref_coloring_available(), reason="reference tree not present (oracle/_ref not built)")
+@pytest.mark.parametrize("uniform", [True, False])
+def test_coloring_equals_reference_graph_coloring(uniform):
+    for name, n, pairs in _graphs():
+        start, ids = _csr_from_pairs(n, pairs)
+        assert np.array_equal(P.color_graph(n, start, ids, uniform), oracle.ref_coloring(n, start, ids, uniform)), name
+    b = read_bfm(GOLDEN / "LinearChain_l1024_n64.bfm.gz")
+    start, ids = b.adjacency()
+    assert np.array_equal(P.color_graph(b.n, start, ids), oracle.ref_coloring(b.n, start, ids))
+
+
+def test_hydrogel_needs_more_than_two_species():
+    b = read_bfm(GOLDEN / "Hydrogel_NStar_117_NoPerXYZ64.bfm")
+    start, ids = b.adjacency()
+    # more bonds per monomer than the reference's MAX_CONNECTIVITY of 4
+    assert int(np.diff(start.astype(np.int64)).max()) > 4
+    col = P.color_graph(b.n, start, ids)
+    assert col.max() + 1 >= 2 and _proper(col, start, ids)
+
+
+# ---- BFM files ----------------------------------------------------------------------
+def test_read_reference_fixtures():
+    lc = read_bfm(GOLDEN / "LinearChain_l1024_n64.bfm.gz")
+    assert lc.n == 65536 and lc.box == (128, 128, 128) and lc.periodic == (True, True, True)
+    assert lc.bonds.shape == (1024 * 63, 2) and len(lc.bondset) == 108
+    assert np.array_equal(np.bincount(lc.attributes), [0, 32768, 32768])
+    ring = read_bfm(GOLDEN / "Ring_N6_n6.bfm")
+    assert ring.n == 42 and ring.periodic == (False, False, False) and ring.bonds.shape[0] == 36
+    gel = read_bfm(GOLDEN / "Hydrogel_NStar_117_NoPerXYZ64.bfm")
+    assert gel.n == 621 and gel.box == (64, 64, 64)
+
+
+def test_linear_chain_fixture_is_a_valid_state():
+    lc = read_bfm(GOLDEN / "LinearChain_l1024_n64.bfm.gz")
+    start, ids = lc.adjacency()
+    o = oracle.System(lc.box, lc.xyz, start, ids, P.color_graph(lc.n, start, ids))
+    assert o.check() == 0
+
+
+def test_bfm_round_trip(tmp_path):
+    for name in ["LinearChain_l1024_n64.bfm.gz", "Ring_N6_n6.bfm", "Hydrogel_NStar_117_NoPerXYZ64.bfm"]:
+        a = read_bfm(GOLDEN / name)
+        out = tmp_path / "rt.bfm"
+        write_bfm(out, a.box, a.xyz, a.bonds, a.bondset, a.attributes, mcs=a.mcs, periodic=a.periodic)
+        b = read_bfm(out)
+        assert np.array_equal(a.xyz, b.xyz) and a.box == b.box and a.periodic == b.periodic
+        key = lambda arr: sorted(tuple(sorted(map(int, p))) for p in arr)
+        assert key(a.bonds) == key(b.bonds)
+        assert np.array_equal(a.attributes, b.attributes)
+
+
+# ---- synthetic systems --------------------------------------------------------------
+def test_generated_systems_are_valid():
+    for s in [chains((64, 64, 64), 64, 64, seed=1), ordered_chains((64, 64, 64), 512, 32),
+              ordered_chains((32, 64, 128), 200, 100), stars((64, 64, 64), 30, 4, 29, seed=2)]:
+        start, ids = s.csr()
+        o = oracle_for(s, P.color_graph(s.n, start, ids), 0)
+        assert o.check() == 0
+
+
+def test_ordered_packing_reaches_half_filling():
+    box = (64, 64, 64)
+    n = 64 ** 3 // 16
+    s = ordered_chains(box, n // 64, 64)
+    assert 8 * s.n / 64 ** 3 == pytest.approx(0.5)
+
+
+def test_saw_growth_is_seeded_and_reports_jams():
+    a = P.gen_linear_chains((32, 32, 32), 10, 20, seed=4)
+    b = P.gen_linear_chains((32, 32, 32), 10, 20, seed=4)
+    assert np.array_equal(a, b)
+    with pytest.raises(RuntimeError):
+        P.gen_linear_chains((16, 16, 16), 200, 20, seed=1, max_restarts=50)
+
+
+def test_star_network_degrees():
+    s = stars((64, 64, 64), 20, 4, 29, seed=3)
+    start, _ = s.csr()
+    deg = np.diff(start.astype(np.int64))
+    assert deg.max() <= 4 and deg[0] == 4
+    assert s.bonds.shape[0] > 20 * 4 * 29   # some crosslinks formed
