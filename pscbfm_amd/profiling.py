@@ -29,7 +29,7 @@ def _short(name: str) -> str:
     for k in KERNELS:
         if k in name:
             return k
-    return name.split("(")[0][-40:]
+    return name.split("(")[0].split("::")[-1][-40:] or name[:40]
 
 
 def _find(d: Path, suffix: str) -> Path:
@@ -100,7 +100,17 @@ def main(argv: Optional[list] = None) -> int:
     lines = [f"# {args.title}", ""]
     if args.command:
         lines += ["Commands (run from the repository root on one MI355X):", "", "```", args.command, "```", ""]
-    lines += ["| kernel | calls | mean duration (us) | over last | total (us) |", "|---|---|---|---|---|"]
+    stats_csv = sorted(Path(args.stats).rglob("*kernel_stats.csv"))
+    if stats_csv:
+        lines += ["rocprofv3 --stats (all dispatches of the run):", "",
+                  "| kernel | calls | average (us) | min (us) | max (us) | share % |", "|---|---|---|---|---|---|"]
+        for r in csv.DictReader(open(stats_csv[0])):
+            lines.append(f"| {KERNELS.get(_short(r['Name']), _short(r['Name']))} | {r['Calls']} | "
+                         f"{float(r['AverageNs']) / 1e3:.2f} | {float(r['MinNs']) / 1e3:.2f} | "
+                         f"{float(r['MaxNs']) / 1e3:.2f} | {float(r['Percentage']):.1f} |")
+        lines.append("")
+    lines += ["Kernel trace, steady state (mean over the last dispatches of each sweep kernel):", "",
+              "| kernel | calls | mean duration (us) | over last | total (us) |", "|---|---|---|---|---|"]
     for k, v in sorted(dur.items(), key=lambda kv: -kv[1]["total_us"]):
         lines.append(f"| {KERNELS.get(k, k)} | {v['calls']} | {v['mean_us']:.2f} | {v['mean_over']} | "
                      f"{v['total_us']:.1f} |")
