@@ -180,6 +180,58 @@ int pscbfm_gen_star_network(uint32_t lx, uint32_t ly, uint32_t lz, uint32_t n_st
  * to scale profiler byte counters and as the achievable-bandwidth reference. */
 int pscbfm_copy_probe(int gpu, uint64_t bytes, int repeats, double* ms_per_copy);
 
+/* ---- Batched replicas (continuous batching) ----------------------------------
+ * No single counterpart in the reference: it replaces a host loop of
+ * independent UpdaterGPUScBFM_AB_Type instances, each running its own
+ * runSimulationOnGPU (UpdaterGPUScBFM_AB_Type.cu:1030-1122), by one launch per
+ * pass that covers every live replica.  A replica is the system staged in an
+ * updater handle (setLatticeSize/.../setConnectivity and setSeed, no
+ * initialize needed); each replica keeps its own seed, substep counter and
+ * stop point, so its trajectory is bit-identical to the same system run
+ * alone in an updater.  Replicas may be added and removed between launches. */
+typedef struct pscbfm_batch pscbfm_batch;
+
+typedef struct {
+    int32_t live;
+    uint32_t n_colors;
+    uint32_t n_monomers;
+    uint32_t tiles;         /* workgroups per launch */
+    uint64_t substeps;      /* done, as of the last synchronize */
+    uint64_t stop_at;       /* the replica idles from this substep on */
+    uint64_t accepted;      /* accepted moves since it joined */
+    uint64_t device_bytes;  /* its own device buffers */
+} pscbfm_replica_info;
+
+/* tile_capacity: workgroups per launch (rounded up to a multiple of 8), the
+ * sum of the replicas' tiles may not exceed it; replica_capacity: the most
+ * replicas at once; block (64..256, multiple of 64) and slots (1, 2, 4) as in
+ * pscbfm_set_launch_shape; lattice_kind: PSCBFM_LATTICE_*. */
+int pscbfm_batch_create(int gpu, int lattice_kind, uint32_t tile_capacity, uint32_t replica_capacity,
+                        uint32_t block, uint32_t slots, pscbfm_batch** out);
+void pscbfm_batch_destroy(pscbfm_batch* b);
+const char* pscbfm_batch_last_error(const pscbfm_batch* b);
+/* Device bytes and workgroups the system staged in `staged` would take. */
+int pscbfm_batch_footprint(pscbfm_batch* b, const pscbfm_updater* staged, uint64_t* bytes, uint32_t* tiles);
+/* Adds the system staged in `staged` (with its seed, coloring settings and
+ * bond set); the replica idles until a target is set.  PSCBFM_RUNTIME_ERROR
+ * when the tile or replica capacity is exhausted. */
+int pscbfm_batch_add(pscbfm_batch* b, const pscbfm_updater* staged, uint32_t* id);
+int pscbfm_batch_remove(pscbfm_batch* b, uint32_t id);
+/* The replica stops after n_mcs more sweeps (counted from the last synchronize). */
+int pscbfm_batch_set_target(pscbfm_batch* b, uint32_t id, uint64_t n_mcs);
+/* Enqueues `substeps` substeps of every live replica (returns at once). */
+int pscbfm_batch_launch(pscbfm_batch* b, uint64_t substeps);
+/* Waits for queued work and refreshes every replica's counters. */
+int pscbfm_batch_synchronize(pscbfm_batch* b);
+/* Sets every live replica's target to n_mcs more sweeps and runs to it;
+ * gpu_ms (optional) receives the GPU time of the launches. */
+int pscbfm_batch_run(pscbfm_batch* b, uint64_t n_mcs, double* gpu_ms);
+int pscbfm_batch_info(pscbfm_batch* b, uint32_t id, pscbfm_replica_info* info);
+/* Unfolded coordinates of replica `id` in its input order (3*n ints). */
+int pscbfm_batch_get_positions(pscbfm_batch* b, uint32_t id, int32_t* xyz, uint32_t n);
+int pscbfm_batch_usage(pscbfm_batch* b, uint32_t* live, uint32_t* used_tiles, uint32_t* tile_capacity,
+                       uint64_t* device_bytes);
+
 /* Library version string. */
 const char* pscbfm_version(void);
 

@@ -24,7 +24,7 @@ __all__ = [
     "UpdaterGPUScBFM_AB_Type", "InvalidArgument", "DeviceError", "RunStats", "lib", "library_path",
     "LATTICE_U8", "LATTICE_BIT", "standard_bond_vectors", "move_hash", "substep_draw",
     "zcurve_encode", "color_graph", "gen_linear_chains", "gen_linear_chains_ordered",
-    "gen_star_network", "chain_bonds",
+    "gen_star_network", "chain_bonds", "Batch", "ReplicaInfo",
 ]
 
 LATTICE_U8 = 0
@@ -47,6 +47,17 @@ class RunStats(C.Structure):
         ("gpu_ms", C.c_double), ("substeps", C.c_uint64), ("attempted", C.c_uint64),
         ("accepted", C.c_uint64), ("bond_terms", C.c_uint64), ("check_ms", C.c_double),
         ("perform_ms", C.c_double), ("check_launches", C.c_uint64), ("perform_launches", C.c_uint64),
+    ]
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+class ReplicaInfo(C.Structure):
+    _fields_ = [
+        ("live", C.c_int32), ("n_colors", C.c_uint32), ("n_monomers", C.c_uint32), ("tiles", C.c_uint32),
+        ("substeps", C.c_uint64), ("stop_at", C.c_uint64), ("accepted", C.c_uint64),
+        ("device_bytes", C.c_uint64),
     ]
 
     def as_dict(self) -> dict:
@@ -110,6 +121,20 @@ _SIGNATURES = {
     "pscbfm_gen_linear_chains_ordered": (C.c_int, [C.c_uint32] * 5 + [_i32p]),
     "pscbfm_gen_star_network": (C.c_int, [C.c_uint32] * 6 + [C.c_uint64, _i32p, _u32p, C.c_uint64, _u64p]),
     "pscbfm_copy_probe": (C.c_int, [C.c_int, C.c_uint64, C.c_int, C.POINTER(C.c_double)]),
+    "pscbfm_batch_create": (C.c_int, [C.c_int, C.c_int, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
+                                       C.POINTER(_H)]),
+    "pscbfm_batch_destroy": (None, [_H]),
+    "pscbfm_batch_last_error": (C.c_char_p, [_H]),
+    "pscbfm_batch_footprint": (C.c_int, [_H, _H, _u64p, _u32p]),
+    "pscbfm_batch_add": (C.c_int, [_H, _H, _u32p]),
+    "pscbfm_batch_remove": (C.c_int, [_H, C.c_uint32]),
+    "pscbfm_batch_set_target": (C.c_int, [_H, C.c_uint32, C.c_uint64]),
+    "pscbfm_batch_launch": (C.c_int, [_H, C.c_uint64]),
+    "pscbfm_batch_synchronize": (C.c_int, [_H]),
+    "pscbfm_batch_run": (C.c_int, [_H, C.c_uint64, C.POINTER(C.c_double)]),
+    "pscbfm_batch_info": (C.c_int, [_H, C.c_uint32, C.POINTER(ReplicaInfo)]),
+    "pscbfm_batch_get_positions": (C.c_int, [_H, C.c_uint32, _i32p, C.c_uint32]),
+    "pscbfm_batch_usage": (C.c_int, [_H, _u32p, _u32p, _u32p, _u64p]),
     "pscbfm_version": (C.c_char_p, []),
 }
 
@@ -316,6 +341,77 @@ class UpdaterGPUScBFM_AB_Type:
         a, b = C.c_uint64(), C.c_uint64()
         self._call("pscbfm_count_proposal_bits", C.byref(a), C.byref(b))
         return a.value, b.value
+
+
+class Batch:
+    """Many independent systems swept by one launch per pass (include/pscbfm.h
+    pscbfm_batch_*).  A replica is the system staged in an updater (no
+    initialize needed) with that updater's seed; it evolves bit-identically to
+    the same updater run alone.  Replicas join and leave between launches."""
+
+    def __init__(self, tile_capacity: int, replica_capacity: int, lattice_kind: int = LATTICE_BIT,
+                 block: int = 256, slots: int = 2, gpu: int = 0) -> None:
+        self._lib = lib()
+        h = C.c_void_p()
+        code = self._lib.pscbfm_batch_create(int(gpu), int(lattice_kind), int(tile_capacity),
+                                             int(replica_capacity), int(block), int(slots), C.byref(h))
+        _raise(code, (self._lib.pscbfm_create_error() or b"").decode())
+        self._h = h
+
+    def __del__(self) -> None:
+        h = getattr(self, "_h", None)
+        if h:
+            self._lib.pscbfm_batch_destroy(h)
+            self._h = None
+
+    def _call(self, name: str, *args) -> None:
+        code = getattr(self._lib, name)(self._h, *args)
+        if code:
+            _raise(code, (self._lib.pscbfm_batch_last_error(self._h) or b"").decode())
+
+    def footprint(self, staged: "UpdaterGPUScBFM_AB_Type"):
+        """(device bytes, workgroups) the staged system would take."""
+        b, t = C.c_uint64(), C.c_uint32()
+        self._call("pscbfm_batch_footprint", staged._h, C.byref(b), C.byref(t))
+        return b.value, t.value
+
+    def add(self, staged: "UpdaterGPUScBFM_AB_Type") -> int:
+        rid = C.c_uint32()
+        self._call("pscbfm_batch_add", staged._h, C.byref(rid))
+        return rid.value
+
+    def remove(self, rid: int) -> None:
+        self._call("pscbfm_batch_remove", int(rid))
+
+    def set_target(self, rid: int, n_mcs: int) -> None:
+        self._call("pscbfm_batch_set_target", int(rid), int(n_mcs))
+
+    def launch(self, substeps: int) -> None:
+        self._call("pscbfm_batch_launch", int(substeps))
+
+    def synchronize(self) -> None:
+        self._call("pscbfm_batch_synchronize")
+
+    def run(self, n_mcs: int) -> float:
+        ms = C.c_double()
+        self._call("pscbfm_batch_run", int(n_mcs), C.byref(ms))
+        return ms.value
+
+    def info(self, rid: int) -> ReplicaInfo:
+        out = ReplicaInfo()
+        self._call("pscbfm_batch_info", int(rid), C.byref(out))
+        return out
+
+    def positions(self, rid: int, n: int) -> np.ndarray:
+        out = np.empty((n, 3), dtype=np.int32)
+        self._call("pscbfm_batch_get_positions", int(rid), _ptr(out, C.c_int32), int(n))
+        return out
+
+    def usage(self):
+        """(live replicas, used workgroups, workgroup capacity, device bytes)."""
+        a, b, c, d = C.c_uint32(), C.c_uint32(), C.c_uint32(), C.c_uint64()
+        self._call("pscbfm_batch_usage", C.byref(a), C.byref(b), C.byref(c), C.byref(d))
+        return a.value, b.value, c.value, d.value
 
 
 # -- host helpers -----------------------------------------------------------------

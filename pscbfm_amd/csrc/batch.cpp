@@ -1,0 +1,301 @@
+// Batched sweep of many independent replicas (see batch.h).
+//
+// Device tables: one SysDesc per replica slot, one Tile per workgroup of the
+// fixed-size launch grid.  Workgroup w runs on XCD w%8; the tiles of a small
+// replica are all taken from one XCD (its lattice then stays in that XCD's
+// L2), a large replica is spread over all XCDs.  Because the grid never
+// changes, a captured graph stays valid while replicas come and go; only the
+// tables are rewritten between launches.
+#include "batch.h"
+
+#include <algorithm>
+#include <cstring>
+#include <sstream>
+#include <stdexcept>
+
+namespace pscbfm {
+
+namespace {
+constexpr uint32_t kSpreadTiles = 64;  // replicas with at least this many workgroups use every XCD
+
+uint32_t degree_bucket(uint32_t d) { return d <= 2 ? 2u : (d <= 4 ? 4u : 8u); }
+}  // namespace
+
+Batch::Batch(int gpu, uint32_t latticeKind, uint32_t tileCapacity, uint32_t replicaCapacity, uint32_t block,
+             uint32_t slots)
+    : gpu_(gpu), latticeKind_(latticeKind) {
+    if (latticeKind != kLatticeU8 && latticeKind != kLatticeBit) throw std::invalid_argument("unknown lattice kind");
+    if (block == 0 || block % 64 != 0 || block > 256) throw std::invalid_argument("block must be 64..256, multiple of 64");
+    if (slots != 1 && slots != 2 && slots != 4) throw std::invalid_argument("slots must be 1, 2 or 4");
+    if (tileCapacity == 0 || replicaCapacity == 0) throw std::invalid_argument("capacities must be positive");
+    tileCap_ = (tileCapacity + 7) / 8 * 8;
+    replicaCap_ = replicaCapacity;
+    shape_.block = block;
+    shape_.slots = slots;
+    shape_.grid = tileCap_;
+    hip_check(hipSetDevice(gpu_), "hipSetDevice");
+    hip_check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
+    hip_check(hipMalloc(&dDescs_, sizeof(SysDesc) * replicaCap_), "hipMalloc descs");
+    hip_check(hipMalloc(&dTiles_, sizeof(Tile) * tileCap_), "hipMalloc tiles");
+    hip_check(hipMalloc(&dLists_, sizeof(uint2) * uint64_t(tileCap_) * block * slots), "hipMalloc lists");
+    hip_check(hipMalloc(&dListCounts_, sizeof(uint32_t) * tileCap_), "hipMalloc list counts");
+    hip_check(hipMalloc(&dStops_, sizeof(uint64_t) * replicaCap_), "hipMalloc stops");
+    hip_check(hipMalloc(&dGather_, sizeof(uint64_t) * 3 * replicaCap_), "hipMalloc gather");
+    hip_check(hipMemsetAsync(dListCounts_, 0, sizeof(uint32_t) * tileCap_, stream_), "memset");
+    descs_.assign(replicaCap_, SysDesc{});
+    Tile idle{kNoReplica, 0, 0, 0};
+    tiles_.assign(tileCap_, idle);
+    replicas_.resize(replicaCap_);
+    freeSlots_.assign(8, {});
+    for (uint32_t w = tileCap_; w-- > 0;) freeSlots_[w % 8].push_back(w);  // pop_back gives the lowest slot
+    tablesDirty_ = true;
+}
+
+Batch::~Batch() {
+    if (stream_) (void)hipStreamSynchronize(stream_);
+    if (graph_) (void)hipGraphExecDestroy(graph_);
+    for (auto& r : replicas_) r.dev.reset();
+    for (void* p : {static_cast<void*>(dDescs_), static_cast<void*>(dTiles_), static_cast<void*>(dLists_),
+                    static_cast<void*>(dListCounts_), static_cast<void*>(dStops_), static_cast<void*>(dGather_)})
+        if (p) (void)hipFree(p);
+    if (stream_) (void)hipStreamDestroy(stream_);
+}
+
+uint32_t Batch::tilesNeeded(const SystemLayout& L) const {
+    return std::max<uint32_t>(1u, tiles_for(L, shape_.block * shape_.slots));
+}
+
+uint64_t Batch::footprint(const SystemLayout& L) const {
+    return DeviceSystem::footprint(L, latticeKind_, false, tilesNeeded(L));
+}
+
+uint32_t Batch::add(const SystemLayout& L, uint64_t seed) {
+    uint32_t id = 0;
+    while (id < replicaCap_ && replicas_[id].live) ++id;
+    if (id == replicaCap_) throw std::runtime_error("batch: replica capacity exhausted");
+    const uint32_t nT = tilesNeeded(L);
+    if (usedTiles_ + nT > tileCap_) {
+        std::ostringstream msg;
+        msg << "batch: tile capacity exhausted (" << usedTiles_ << " + " << nT << " > " << tileCap_ << ")";
+        throw std::runtime_error(msg.str());
+    }
+    hip_check(hipSetDevice(gpu_), "hipSetDevice");
+    Replica& r = replicas_[id];
+    r = Replica{};
+    r.layout = L;
+    r.seed = seed;
+    r.dev.reset(new DeviceSystem());
+    r.dev->allocate(L, latticeKind_, seed, /*withLists=*/false, nT, stream_);
+    DevState st = r.dev->readState(stream_);
+    st.stopAt = 0;  // idle until a target is set
+    r.dev->writeState(st, stream_);
+
+    // Workgroup slots: one XCD for a small replica, all XCDs for a large one.
+    r.tileSlots.reserve(nT);
+    if (nT >= kSpreadTiles) {
+        uint32_t x = 0;
+        while (r.tileSlots.size() < nT) {
+            for (uint32_t tries = 0; tries < 8 && freeSlots_[x].empty(); ++tries) x = (x + 1) % 8;
+            r.tileSlots.push_back(freeSlots_[x].back());
+            freeSlots_[x].pop_back();
+            x = (x + 1) % 8;
+        }
+    } else {
+        while (r.tileSlots.size() < nT) {
+            uint32_t best = 0;
+            for (uint32_t x = 1; x < 8; ++x)
+                if (freeSlots_[x].size() > freeSlots_[best].size()) best = x;
+            const uint32_t take = std::min<uint32_t>(nT - uint32_t(r.tileSlots.size()), uint32_t(freeSlots_[best].size()));
+            for (uint32_t k = 0; k < take; ++k) {
+                r.tileSlots.push_back(freeSlots_[best].back());
+                freeSlots_[best].pop_back();
+            }
+        }
+    }
+    for (uint32_t c = 0; c < nT; ++c) tiles_[r.tileSlots[c]] = Tile{id, c, nT, 0};
+    usedTiles_ += nT;
+    descs_[id] = r.dev->desc;
+    r.live = true;
+    r.pendingStop = 0;
+    r.info = BatchReplicaInfo{};
+    r.info.live = true;
+    r.info.nColors = uint32_t(L.colors.size());
+    r.info.nMonomers = L.nMonomers;
+    r.info.tiles = nT;
+    r.info.deviceBytes = r.dev->bytes;
+    const uint32_t bucket = degree_bucket(r.dev->desc.maxDegree);
+    if (bucket > graphDegree_) {
+        graphDegree_ = bucket;
+        if (graph_) {
+            (void)hipGraphExecDestroy(graph_);
+            graph_ = nullptr;
+        }
+    }
+    tablesDirty_ = true;
+    return id;
+}
+
+void Batch::remove(uint32_t id) {
+    if (id >= replicaCap_ || !replicas_[id].live) throw std::invalid_argument("batch: no such replica");
+    hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+    Replica& r = replicas_[id];
+    for (uint32_t w : r.tileSlots) {
+        tiles_[w] = Tile{kNoReplica, 0, 0, 0};
+        freeSlots_[w % 8].push_back(w);
+    }
+    for (auto& f : freeSlots_) std::sort(f.begin(), f.end(), std::greater<uint32_t>());
+    usedTiles_ -= uint32_t(r.tileSlots.size());
+    descs_[id] = SysDesc{};
+    // the tables must not point at freed memory when the next launch runs
+    tablesDirty_ = true;
+    uploadTables();
+    hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+    r.dev.reset();
+    r = Replica{};
+}
+
+void Batch::setTarget(uint32_t id, uint64_t nMCS) {
+    if (id >= replicaCap_ || !replicas_[id].live) throw std::invalid_argument("batch: no such replica");
+    Replica& r = replicas_[id];
+    r.pendingStop = r.info.substep + nMCS * r.info.nColors;
+    r.info.stopAt = r.pendingStop;
+    stopsDirty_ = true;
+}
+
+void Batch::uploadTables() {
+    if (tablesDirty_) {
+        hip_check(hipMemcpyAsync(dDescs_, descs_.data(), sizeof(SysDesc) * replicaCap_, hipMemcpyHostToDevice, stream_),
+                  "descs H2D");
+        hip_check(hipMemcpyAsync(dTiles_, tiles_.data(), sizeof(Tile) * tileCap_, hipMemcpyHostToDevice, stream_),
+                  "tiles H2D");
+        // the host tables may change as soon as this returns
+        hip_check(hipStreamSynchronize(stream_), "sync tables");
+        tablesDirty_ = false;
+    }
+    if (stopsDirty_) {
+        std::vector<uint64_t> stops(replicaCap_, 0);
+        for (uint32_t i = 0; i < replicaCap_; ++i)
+            if (replicas_[i].live) stops[i] = replicas_[i].pendingStop;
+        hip_check(hipMemcpyAsync(dStops_, stops.data(), sizeof(uint64_t) * replicaCap_, hipMemcpyHostToDevice, stream_),
+                  "stops H2D");
+        hip_check(launch_set_stops(dDescs_, replicaCap_, dStops_, stream_), "set stops");
+        hip_check(hipStreamSynchronize(stream_), "sync stops");  // `stops` is a local buffer
+        stopsDirty_ = false;
+    }
+}
+
+void Batch::captureGraph() {
+    if (graph_) {
+        (void)hipGraphExecDestroy(graph_);
+        graph_ = nullptr;
+    }
+    graphSubsteps_ = 16;
+    BatchLaunch b{dDescs_, dTiles_, dLists_, dListCounts_, tileCap_, graphDegree_, latticeKind_};
+    hipGraph_t g = nullptr;
+    hip_check(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+    hipError_t err = hipSuccess;
+    for (uint32_t k = 0; k < graphSubsteps_ && err == hipSuccess; ++k) {
+        err = launch_check_batch(b, k, shape_, stream_);
+        if (err == hipSuccess) err = launch_perform_batch(b, k, shape_, stream_);
+    }
+    if (err == hipSuccess) err = launch_advance_batch(dDescs_, replicaCap_, graphSubsteps_, stream_);
+    const hipError_t endErr = hipStreamEndCapture(stream_, &g);
+    if (err != hipSuccess) hip_fail(err, "kernel launch during capture");
+    hip_check(endErr, "hipStreamEndCapture");
+    const hipError_t ie = hipGraphInstantiate(&graph_, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    hip_check(ie, "hipGraphInstantiate");
+}
+
+void Batch::launch(uint64_t substeps) {
+    hip_check(hipSetDevice(gpu_), "hipSetDevice");
+    uploadTables();
+    if (liveReplicas() == 0 || substeps == 0) return;
+    if (!graph_) captureGraph();
+    uint64_t left = substeps;
+    while (left >= graphSubsteps_) {
+        hip_check(hipGraphLaunch(graph_, stream_), "hipGraphLaunch");
+        left -= graphSubsteps_;
+    }
+    if (left) {
+        BatchLaunch b{dDescs_, dTiles_, dLists_, dListCounts_, tileCap_, graphDegree_, latticeKind_};
+        for (uint32_t k = 0; k < left; ++k) {
+            hip_check(launch_check_batch(b, k, shape_, stream_), "launch pass 1");
+            hip_check(launch_perform_batch(b, k, shape_, stream_), "launch pass 2");
+        }
+        hip_check(launch_advance_batch(dDescs_, replicaCap_, uint32_t(left), stream_), "advance");
+    }
+    launched_ += substeps;
+}
+
+void Batch::synchronize() {
+    hip_check(hipSetDevice(gpu_), "hipSetDevice");
+    uploadTables();
+    hip_check(launch_gather_state(dDescs_, replicaCap_, dGather_, stream_), "gather");
+    std::vector<uint64_t> g(3 * size_t(replicaCap_));
+    hip_check(hipMemcpyAsync(g.data(), dGather_, g.size() * 8, hipMemcpyDeviceToHost, stream_), "gather D2H");
+    hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+    for (uint32_t i = 0; i < replicaCap_; ++i) {
+        Replica& r = replicas_[i];
+        if (!r.live) continue;
+        r.info.substep = g[3 * i];
+        r.info.stopAt = g[3 * i + 1];
+        r.info.accepted = g[3 * i + 2];
+    }
+}
+
+double Batch::run(uint64_t nMCS) {
+    uint64_t need = 0;
+    for (uint32_t i = 0; i < replicaCap_; ++i)
+        if (replicas_[i].live) {
+            setTarget(i, nMCS);
+            need = std::max(need, replicas_[i].pendingStop - replicas_[i].info.substep);
+        }
+    uploadTables();
+    hipEvent_t e0, e1;
+    hip_check(hipEventCreate(&e0), "hipEventCreate");
+    hip_check(hipEventCreate(&e1), "hipEventCreate");
+    hip_check(hipEventRecord(e0, stream_), "hipEventRecord");
+    launch(need);
+    hip_check(hipEventRecord(e1, stream_), "hipEventRecord");
+    hip_check(hipEventSynchronize(e1), "hipEventSynchronize");
+    float ms = 0;
+    hip_check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    synchronize();
+    for (uint32_t i = 0; i < replicaCap_; ++i)
+        if (replicas_[i].live && replicas_[i].info.substep != replicas_[i].info.stopAt)
+            throw std::runtime_error("batch: a replica did not reach its target");
+    return ms;
+}
+
+const BatchReplicaInfo& Batch::info(uint32_t id) const {
+    if (id >= replicaCap_ || !replicas_[id].live) throw std::invalid_argument("batch: no such replica");
+    return replicas_[id].info;
+}
+
+void Batch::positions(uint32_t id, int32_t* xyz, uint32_t n) {
+    if (id >= replicaCap_ || !replicas_[id].live) throw std::invalid_argument("batch: no such replica");
+    const Replica& r = replicas_[id];
+    if (n != r.layout.nMonomers) throw std::invalid_argument("batch: count differs from the replica's monomers");
+    hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+    r.dev->positions(r.layout, xyz, stream_);
+}
+
+uint32_t Batch::liveReplicas() const {
+    uint32_t n = 0;
+    for (const auto& r : replicas_) n += r.live ? 1u : 0u;
+    return n;
+}
+
+uint64_t Batch::deviceBytes() const {
+    uint64_t b = sizeof(SysDesc) * uint64_t(replicaCap_) + sizeof(Tile) * uint64_t(tileCap_) +
+                 sizeof(uint2) * uint64_t(tileCap_) * shape_.block * shape_.slots + 4ull * tileCap_ +
+                 32ull * replicaCap_;
+    for (const auto& r : replicas_)
+        if (r.live) b += r.dev->bytes;
+    return b;
+}
+
+}  // namespace pscbfm

@@ -3,11 +3,13 @@
 #include "pscbfm.h"
 
 #include <cstring>
+#include <memory>
 #include <new>
 #include <stdexcept>
 #include <string>
 #include <vector>
 
+#include "batch.h"
 #include "coloring.h"
 #include "common.h"
 #include "generators.h"
@@ -18,36 +20,53 @@ struct pscbfm_updater {
     std::string error;
 };
 
+struct pscbfm_batch {
+    std::unique_ptr<pscbfm::Batch> engine;
+    std::string error;
+};
+
 namespace {
 
 thread_local std::string g_createError;
 
+// Runs f, turning its exceptions into a status code and a message.
+template <class F>
+int translated(std::string& error, F&& f) {
+    error.clear();
+    try {
+        f();
+        return PSCBFM_OK;
+    } catch (const std::invalid_argument& e) {
+        error = e.what();
+        return PSCBFM_INVALID_ARGUMENT;
+    } catch (const std::out_of_range& e) {
+        error = e.what();
+        return PSCBFM_OUT_OF_RANGE;
+    } catch (const std::runtime_error& e) {
+        error = e.what();
+        return std::strncmp(e.what(), "HIP error", 9) == 0 ? PSCBFM_DEVICE_ERROR : PSCBFM_RUNTIME_ERROR;
+    } catch (const std::bad_alloc&) {
+        error = "host allocation failed";
+        return PSCBFM_RUNTIME_ERROR;
+    } catch (const std::exception& e) {
+        error = e.what();
+        return PSCBFM_INTERNAL_ERROR;
+    } catch (...) {
+        error = "unknown exception";
+        return PSCBFM_INTERNAL_ERROR;
+    }
+}
+
 template <class F>
 int guarded(pscbfm_updater* h, F&& f) {
     if (!h) return PSCBFM_NULL_HANDLE;
-    h->error.clear();
-    try {
-        f(h->engine);
-        return PSCBFM_OK;
-    } catch (const std::invalid_argument& e) {
-        h->error = e.what();
-        return PSCBFM_INVALID_ARGUMENT;
-    } catch (const std::out_of_range& e) {
-        h->error = e.what();
-        return PSCBFM_OUT_OF_RANGE;
-    } catch (const std::runtime_error& e) {
-        h->error = e.what();
-        return std::strncmp(e.what(), "HIP error", 9) == 0 ? PSCBFM_DEVICE_ERROR : PSCBFM_RUNTIME_ERROR;
-    } catch (const std::bad_alloc&) {
-        h->error = "host allocation failed";
-        return PSCBFM_RUNTIME_ERROR;
-    } catch (const std::exception& e) {
-        h->error = e.what();
-        return PSCBFM_INTERNAL_ERROR;
-    } catch (...) {
-        h->error = "unknown exception";
-        return PSCBFM_INTERNAL_ERROR;
-    }
+    return translated(h->error, [&] { f(h->engine); });
+}
+
+template <class F>
+int batch_guarded(pscbfm_batch* b, F&& f) {
+    if (!b || !b->engine) return PSCBFM_NULL_HANDLE;
+    return translated(b->error, [&] { f(*b->engine); });
 }
 
 template <class F>
@@ -415,6 +434,99 @@ int pscbfm_copy_probe(int gpu, uint64_t bytes, int repeats, double* ms_per_copy)
     if (dst) (void)hipFree(dst);
     if (st) (void)hipStreamDestroy(st);
     return rc;
+}
+
+// ---- batch ------------------------------------------------------------------------
+
+int pscbfm_batch_create(int gpu, int lattice_kind, uint32_t tile_capacity, uint32_t replica_capacity,
+                        uint32_t block, uint32_t slots, pscbfm_batch** out) {
+    if (!out) return PSCBFM_INVALID_ARGUMENT;
+    *out = nullptr;
+    if (gpu < 0) {
+        g_createError = "gpu must be >= 0";
+        return PSCBFM_INVALID_ARGUMENT;
+    }
+    std::unique_ptr<pscbfm_batch> b(new (std::nothrow) pscbfm_batch());
+    if (!b) return PSCBFM_RUNTIME_ERROR;
+    const int rc = translated(g_createError, [&] {
+        b->engine.reset(new pscbfm::Batch(gpu, uint32_t(lattice_kind), tile_capacity, replica_capacity, block, slots));
+    });
+    if (rc == PSCBFM_OK) *out = b.release();
+    return rc;
+}
+
+void pscbfm_batch_destroy(pscbfm_batch* b) { delete b; }
+
+const char* pscbfm_batch_last_error(const pscbfm_batch* b) { return b ? b->error.c_str() : "null handle"; }
+
+int pscbfm_batch_footprint(pscbfm_batch* b, const pscbfm_updater* staged, uint64_t* bytes, uint32_t* tiles) {
+    if (!staged) return PSCBFM_NULL_HANDLE;
+    return batch_guarded(b, [&](pscbfm::Batch& e) {
+        const pscbfm::SystemLayout L = staged->engine.buildLayout();
+        if (bytes) *bytes = e.footprint(L);
+        if (tiles) *tiles = e.tilesNeeded(L);
+    });
+}
+
+int pscbfm_batch_add(pscbfm_batch* b, const pscbfm_updater* staged, uint32_t* id) {
+    if (!staged) return PSCBFM_NULL_HANDLE;
+    return batch_guarded(b, [&](pscbfm::Batch& e) {
+        const uint32_t r = e.add(staged->engine.buildLayout(), staged->engine.seed());
+        if (id) *id = r;
+    });
+}
+
+int pscbfm_batch_remove(pscbfm_batch* b, uint32_t id) {
+    return batch_guarded(b, [&](pscbfm::Batch& e) { e.remove(id); });
+}
+
+int pscbfm_batch_set_target(pscbfm_batch* b, uint32_t id, uint64_t n_mcs) {
+    return batch_guarded(b, [&](pscbfm::Batch& e) { e.setTarget(id, n_mcs); });
+}
+
+int pscbfm_batch_launch(pscbfm_batch* b, uint64_t substeps) {
+    return batch_guarded(b, [&](pscbfm::Batch& e) { e.launch(substeps); });
+}
+
+int pscbfm_batch_synchronize(pscbfm_batch* b) {
+    return batch_guarded(b, [&](pscbfm::Batch& e) { e.synchronize(); });
+}
+
+int pscbfm_batch_run(pscbfm_batch* b, uint64_t n_mcs, double* gpu_ms) {
+    return batch_guarded(b, [&](pscbfm::Batch& e) {
+        const double ms = e.run(n_mcs);
+        if (gpu_ms) *gpu_ms = ms;
+    });
+}
+
+int pscbfm_batch_info(pscbfm_batch* b, uint32_t id, pscbfm_replica_info* info) {
+    if (!info) return PSCBFM_INVALID_ARGUMENT;
+    return batch_guarded(b, [&](pscbfm::Batch& e) {
+        const pscbfm::BatchReplicaInfo& r = e.info(id);
+        info->live = r.live ? 1 : 0;
+        info->n_colors = r.nColors;
+        info->n_monomers = r.nMonomers;
+        info->tiles = r.tiles;
+        info->substeps = r.substep;
+        info->stop_at = r.stopAt;
+        info->accepted = r.accepted;
+        info->device_bytes = r.deviceBytes;
+    });
+}
+
+int pscbfm_batch_get_positions(pscbfm_batch* b, uint32_t id, int32_t* xyz, uint32_t n) {
+    if (!xyz && n) return PSCBFM_INVALID_ARGUMENT;
+    return batch_guarded(b, [&](pscbfm::Batch& e) { e.positions(id, xyz, n); });
+}
+
+int pscbfm_batch_usage(pscbfm_batch* b, uint32_t* live, uint32_t* used_tiles, uint32_t* tile_capacity,
+                       uint64_t* device_bytes) {
+    return batch_guarded(b, [&](pscbfm::Batch& e) {
+        if (live) *live = e.liveReplicas();
+        if (used_tiles) *used_tiles = e.usedTiles();
+        if (tile_capacity) *tile_capacity = e.tileCapacity();
+        if (device_bytes) *device_bytes = e.deviceBytes();
+    });
 }
 
 const char* pscbfm_version(void) { return "pscbfm-mi355x 0.1 (gfx950)"; }

@@ -199,6 +199,16 @@ struct DevState {
     uint64_t pad_[2];
 };
 
+// Workgroup assignment of a batched launch: chunk `chunk` of the `chunks`
+// workgroups that cover replica `replica` (kNoReplica: idle workgroup).
+constexpr uint32_t kNoReplica = 0xFFFFFFFFu;
+struct Tile {
+    uint32_t replica;
+    uint32_t chunk;
+    uint32_t chunks;
+    uint32_t pad_;
+};
+
 // Everything a sweep kernel needs to know about one system.
 struct SysDesc {
     uint32_t* pos;           // Z-curve code per slot
@@ -220,6 +230,8 @@ struct SysDesc {
     uint32_t latticeKind;
     uint32_t maxColorCount;  // largest species
     uint32_t maxDegree;      // largest bond count of any monomer
+    uint32_t accEntries;     // entries of state->acceptedByBlock
+    uint32_t pad_;
 };
 
 }  // namespace pscbfm

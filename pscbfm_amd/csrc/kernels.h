@@ -23,6 +23,25 @@ hipError_t launch_check(const SysDesc& d, uint32_t k, const LaunchShape& s, hipS
 hipError_t launch_perform(const SysDesc& d, uint32_t k, const LaunchShape& s, hipStream_t st);
 // DevState::substep += count (after a block of substeps).
 hipError_t launch_advance(DevState* state, uint32_t count, hipStream_t st);
+
+// A batch of independent systems swept by one launch per pass.
+struct BatchLaunch {
+    const SysDesc* descs;     // device table of replica descriptors
+    const Tile* tileTable;    // device, one entry per workgroup
+    uint2* lists;             // tiles * slots * block entries
+    uint32_t* listCounts;     // tiles entries
+    uint32_t tiles;           // workgroups per launch (multiple of 8)
+    uint32_t maxDegree;       // largest bond count over all replicas
+    uint32_t latticeKind;     // the same for every replica
+};
+hipError_t launch_check_batch(const BatchLaunch& b, uint32_t k, const LaunchShape& s, hipStream_t st);
+hipError_t launch_perform_batch(const BatchLaunch& b, uint32_t k, const LaunchShape& s, hipStream_t st);
+// For every replica with a state: substep = min(substep + count, stopAt).
+hipError_t launch_advance_batch(const SysDesc* descs, uint32_t nDescs, uint32_t count, hipStream_t st);
+// stopAt of replica r = stops[r] (replicas without a state are skipped).
+hipError_t launch_set_stops(const SysDesc* descs, uint32_t nDescs, const uint64_t* stops, hipStream_t st);
+// out[3r..3r+2] = substep, stopAt, accepted total of replica r.
+hipError_t launch_gather_state(const SysDesc* descs, uint32_t nDescs, uint64_t* out, hipStream_t st);
 // Fills n 32-bit words with v (graph-safe replacement of a memset).
 hipError_t launch_fill_u32(uint32_t* p, uint64_t n, uint32_t v, hipStream_t st);
 // Streaming copy of n16 16-byte elements (bandwidth and counter reference).

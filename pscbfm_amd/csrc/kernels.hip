@@ -9,21 +9,24 @@
 //    and every neighbouring cell is a masked add on one axis' bits;
 //  * the forbidden-bond table is a 512-bit mask in LDS, indexed by the
 //    Z-order bond key of two codes;
-//  * both passes are bound by memory latency, not bandwidth: a pass-1 thread
-//    handles SLOTS monomers and issues all their loads in two rounds (own
-//    codes and partner ids; then partner codes and the nine front-plane
-//    cells together), so each round trip carries SLOTS times more requests;
+//  * both passes are bound by the number of gather instructions and memory
+//    latency more than by bandwidth: a pass-1 thread handles SLOTS monomers
+//    and issues all their loads in two rounds (own codes and partner ids;
+//    then partner codes and the front plane together, from one basic block);
+//    the 3x3 front plane is read as at most four 64-bit words;
 //  * pass 1 compacts the surviving proposals of its workgroup into a list
 //    (wave ballot + LDS counter; the order inside a list is irrelevant
 //    because the result of a substep does not depend on processing order),
-//    and pass 2 walks those lists densely: one load of (index, direction,
-//    code), one round of nine proposal-lattice loads, then the stores;
+//    and pass 2 walks those lists densely;
 //  * the proposal lattice is 1 bit per cell and double-buffered by substep
 //    parity: pass 1 of substep t sets bits in buffer t&1 and zeroes buffer
 //    (t+1)&1, which pass 2 of substep t-1 was the last to read;
-//  * workgroup w runs on XCD w%8, so logical blocks are renumbered to give
-//    each XCD one contiguous eighth of the species (its own L2 then holds the
-//    lattice region of that eighth);
+//  * the occupancy lattice is updated with plain byte stores (no atomics):
+//    device-scope atomics are executed beyond the per-XCD L2 and cost a
+//    memory transaction each;
+//  * workgroup w runs on XCD w%8: a single system renumbers workgroups so
+//    each XCD takes one contiguous eighth of the species; a batch keeps all
+//    workgroups of a replica on one XCD through its tile table;
 //  * the species and kernel seed of a substep are drawn on the device from
 //    the substep counter, so launches carry no per-substep host data.
 #include "kernels.h"
@@ -106,17 +109,13 @@ __device__ __forceinline__ uint32_t lane_rank(uint64_t mask) {
 // List entry: x = species-local index | direction << 29, y = old code.
 constexpr uint32_t kDirShift = 29;
 
+// Pass 1 of substep t for logical workgroup lb of nb of one system.  The
+// survivors go to `list` (capacity SLOTS * blockDim), their number to *count.
 template <int LAT, int SLOTS, int MAXDEG>
-__global__ __launch_bounds__(256) void k_check(SysDesc d, uint32_t k) {
-    __shared__ uint32_t sForbidden[16];
-    __shared__ uint32_t sCount;
-    const DevState* st = d.state;
-    const uint64_t t = st->substep + k;
-    if (t >= st->stopAt) return;
+__device__ __forceinline__ void check_body(const SysDesc& d, uint64_t t, uint32_t lb, uint32_t nb, uint2* list,
+                                           uint32_t* count, uint32_t* sForbidden, uint32_t* sCount) {
     const SubstepDraw draw = substep_draw(d.seed, t, d.nColors);
     const ColorInfo ci = d.colors[draw.species];
-    const uint32_t lb = xcd_block();
-    const uint32_t nb = gridDim.x;
 
     // Zero the proposal buffer of the other parity, one slice per workgroup.
     {
@@ -128,7 +127,7 @@ __global__ __launch_bounds__(256) void k_check(SysDesc d, uint32_t k) {
         for (uint32_t q = q0 + threadIdx.x; q < q1; q += blockDim.x) prev[q] = make_uint4(0u, 0u, 0u, 0u);
     }
     if (threadIdx.x < 16) sForbidden[threadIdx.x] = d.bondMask[threadIdx.x];
-    if (threadIdx.x == 0) sCount = 0;
+    if (threadIdx.x == 0) *sCount = 0;
     __syncthreads();
 
     const uint32_t B = blockDim.x;
@@ -182,13 +181,12 @@ __global__ __launch_bounds__(256) void k_check(SysDesc d, uint32_t k) {
 
     // Mark targets and append survivors to this workgroup's list.
     uint32_t* T = d.proposal + size_t(t & 1u) * d.latWords;
-    uint2* list = d.list + size_t(lb) * (SLOTS * B);
     const uint32_t lane = threadIdx.x & 63u;
 #pragma unroll
     for (int s = 0; s < SLOTS; ++s) {
         const uint64_t mask = __ballot(ok[s]);
         uint32_t waveBase = 0;
-        if (lane == 0 && mask != 0) waveBase = atomicAdd(&sCount, uint32_t(__popcll(mask)));
+        if (lane == 0 && mask != 0) waveBase = atomicAdd(sCount, uint32_t(__popcll(mask)));
         waveBase = __shfl(waveBase, 0, 64);
         if (ok[s]) {
             atomicOr(&T[to[s] >> 5], 1u << (to[s] & 31u));
@@ -196,7 +194,7 @@ __global__ __launch_bounds__(256) void k_check(SysDesc d, uint32_t k) {
         }
     }
     __syncthreads();
-    if (threadIdx.x == 0) d.listCount[lb] = sCount;
+    if (threadIdx.x == 0) *count = *sCount;
 }
 
 __device__ __forceinline__ uint32_t block_sum(uint32_t v, uint32_t* scratch) {
@@ -211,23 +209,19 @@ __device__ __forceinline__ uint32_t block_sum(uint32_t v, uint32_t* scratch) {
     return total;
 }
 
-template <int LAT, int SLOTS>
-__global__ __launch_bounds__(256) void k_perform(SysDesc d, uint32_t k) {
-    __shared__ uint32_t sSum[4];
+// Pass 2 of substep t over one pass-1 list.  The accepted count is added to
+// *accSlot, an entry only this workgroup writes.
+template <int LAT>
+__device__ __forceinline__ void perform_body(const SysDesc& d, uint64_t t, const uint2* list, uint32_t count,
+                                             uint64_t* accSlot, uint32_t* sSum) {
     DevState* st = d.state;
-    const uint64_t t = st->substep + k;
-    if (t >= st->stopAt) return;
     const SubstepDraw draw = substep_draw(d.seed, t, d.nColors);
     const ColorInfo ci = d.colors[draw.species];
-    const uint32_t lb = xcd_block();  // same numbering as pass 1: list lb
     const uint32_t B = blockDim.x;
-    const uint32_t count = d.listCount[lb];
-    const uint2* list = d.list + size_t(lb) * (SLOTS * B);
-    const uint32_t* T = d.proposal + size_t(t & 1u) * d.latWords;
+    const uint64_t* T64 = reinterpret_cast<const uint64_t*>(d.proposal + size_t(t & 1u) * d.latWords);
 
     // The list is compacted, so about a fifth of its capacity is filled: walk
     // only the entries that exist, one per thread and round.
-    const uint64_t* T64 = reinterpret_cast<const uint64_t*>(T);
     uint32_t accepted = 0;
     for (uint32_t idx = threadIdx.x; idx < count; idx += B) {
         const uint2 e = list[idx];
@@ -264,7 +258,7 @@ __global__ __launch_bounds__(256) void k_perform(SysDesc d, uint32_t k) {
     }
     const uint32_t total = block_sum(accepted, sSum);
     if (threadIdx.x == 0 && total != 0u) {
-        st->acceptedByBlock[blockIdx.x] += total;  // this workgroup's own entry
+        *accSlot += total;
         if (st->log != nullptr) {
             const uint64_t idx = t - st->logFirst;
             if (idx < st->logCapacity) atomicAdd(&st->log[idx], total);
@@ -272,9 +266,101 @@ __global__ __launch_bounds__(256) void k_perform(SysDesc d, uint32_t k) {
     }
 }
 
+// ---- single system ----------------------------------------------------------------
+template <int LAT, int SLOTS, int MAXDEG>
+__global__ __launch_bounds__(256) void k_check(SysDesc d, uint32_t k) {
+    __shared__ uint32_t sForbidden[16];
+    __shared__ uint32_t sCount;
+    const uint64_t t = d.state->substep + k;
+    if (t >= d.state->stopAt) return;
+    const uint32_t lb = xcd_block();
+    check_body<LAT, SLOTS, MAXDEG>(d, t, lb, gridDim.x, d.list + size_t(lb) * (SLOTS * blockDim.x),
+                                   &d.listCount[lb], sForbidden, &sCount);
+}
+
+template <int LAT, int SLOTS>
+__global__ __launch_bounds__(256) void k_perform(SysDesc d, uint32_t k) {
+    __shared__ uint32_t sSum[4];
+    const uint64_t t = d.state->substep + k;
+    if (t >= d.state->stopAt) return;
+    const uint32_t lb = xcd_block();  // same numbering as pass 1: list lb
+    perform_body<LAT>(d, t, d.list + size_t(lb) * (SLOTS * blockDim.x), d.listCount[lb],
+                      &d.state->acceptedByBlock[blockIdx.x], sSum);
+}
+
+// ---- batch of replicas ---------------------------------------------------------------
+// Workgroup w serves tiles[w]: chunk `chunk` of `chunks` of replica `replica`.
+// Lists are indexed by w, accepted counts by (replica, chunk).
+template <int LAT, int SLOTS, int MAXDEG>
+__global__ __launch_bounds__(256) void k_check_batch(const SysDesc* __restrict__ descs, const Tile* __restrict__ tiles,
+                                                     uint2* lists, uint32_t* listCounts, uint32_t k) {
+    __shared__ uint32_t sForbidden[16];
+    __shared__ uint32_t sCount;
+    const Tile tile = tiles[blockIdx.x];
+    if (tile.replica == kNoReplica) return;
+    const SysDesc& d = descs[tile.replica];
+    const uint64_t t = d.state->substep + k;
+    if (t >= d.state->stopAt) return;
+    check_body<LAT, SLOTS, MAXDEG>(d, t, tile.chunk, tile.chunks,
+                                   lists + size_t(blockIdx.x) * (SLOTS * blockDim.x), &listCounts[blockIdx.x],
+                                   sForbidden, &sCount);
+}
+
+template <int LAT, int SLOTS>
+__global__ __launch_bounds__(256) void k_perform_batch(const SysDesc* __restrict__ descs,
+                                                       const Tile* __restrict__ tiles, const uint2* lists,
+                                                       const uint32_t* listCounts, uint32_t k) {
+    __shared__ uint32_t sSum[4];
+    const Tile tile = tiles[blockIdx.x];
+    if (tile.replica == kNoReplica) return;
+    const SysDesc& d = descs[tile.replica];
+    const uint64_t t = d.state->substep + k;
+    if (t >= d.state->stopAt) return;
+    perform_body<LAT>(d, t, lists + size_t(blockIdx.x) * (SLOTS * blockDim.x), listCounts[blockIdx.x],
+                      &d.state->acceptedByBlock[tile.chunk], sSum);
+}
+
 __global__ void k_advance(DevState* st, uint32_t count) {
     if (blockIdx.x == 0 && threadIdx.x == 0)
         atomicAdd(reinterpret_cast<unsigned long long*>(&st->substep), (unsigned long long)count);
+}
+
+// Every live replica: substep = min(substep + count, stopAt).
+__global__ void k_advance_batch(const SysDesc* descs, uint32_t nDescs, uint32_t count) {
+    const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nDescs) return;
+    DevState* st = descs[r].state;
+    if (st == nullptr) return;
+    const uint64_t next = st->substep + count;
+    st->substep = next < st->stopAt ? next : (st->substep > st->stopAt ? st->substep : st->stopAt);
+}
+
+__global__ void k_set_stops(const SysDesc* descs, uint32_t nDescs, const uint64_t* stops) {
+    const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nDescs || descs[r].state == nullptr) return;
+    descs[r].state->stopAt = stops[r];
+}
+
+// One workgroup per replica sums its per-workgroup accepted counters.
+__global__ void k_gather_state(const SysDesc* descs, uint32_t nDescs, uint64_t* out) {
+    __shared__ uint64_t part[256];
+    const uint32_t r = blockIdx.x;
+    if (r >= nDescs) return;
+    const DevState* st = descs[r].state;
+    uint64_t sum = 0;
+    if (st != nullptr)
+        for (uint32_t i = threadIdx.x; i < descs[r].accEntries; i += blockDim.x) sum += st->acceptedByBlock[i];
+    part[threadIdx.x] = sum;
+    __syncthreads();
+    for (uint32_t w = blockDim.x / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w) part[threadIdx.x] += part[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        out[3 * r + 0] = st ? st->substep : 0;
+        out[3 * r + 1] = st ? st->stopAt : 0;
+        out[3 * r + 2] = part[0];
+    }
 }
 
 __global__ void k_fill_u32(uint32_t* p, uint64_t n, uint32_t v) {
@@ -283,52 +369,119 @@ __global__ void k_fill_u32(uint32_t* p, uint64_t n, uint32_t v) {
         p[i] = v;
 }
 
-template <int LAT, int SLOTS>
-hipError_t check_by_degree(const SysDesc& d, uint32_t k, const LaunchShape& s, hipStream_t st) {
-    const dim3 g(s.grid), b(s.block);
-    if (d.maxDegree <= 2)
-        hipLaunchKernelGGL((k_check<LAT, SLOTS, 2>), g, b, 0, st, d, k);
-    else if (d.maxDegree <= 4)
-        hipLaunchKernelGGL((k_check<LAT, SLOTS, 4>), g, b, 0, st, d, k);
+// Plain streaming copy: the reference point that scales the HBM byte
+// counters of a profiler run (it reads and writes exactly n*16 bytes).
+__global__ void k_copy_u4(const uint4* __restrict__ src, uint4* __restrict__ dst, uint64_t n) {
+    for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += uint64_t(gridDim.x) * blockDim.x)
+        dst[i] = src[i];
+}
+
+// ---- template dispatch ------------------------------------------------------------------
+template <int LAT, int SLOTS, int MAXDEG>
+void check_launch(const SysDesc& d, const BatchLaunch* b, uint32_t k, const LaunchShape& s, hipStream_t st) {
+    const dim3 blk(s.block);
+    if (b)
+        hipLaunchKernelGGL((k_check_batch<LAT, SLOTS, MAXDEG>), dim3(b->tiles), blk, 0, st, b->descs, b->tileTable,
+                           b->lists, b->listCounts, k);
     else
-        hipLaunchKernelGGL((k_check<LAT, SLOTS, 8>), g, b, 0, st, d, k);
-    return hipGetLastError();
+        hipLaunchKernelGGL((k_check<LAT, SLOTS, MAXDEG>), dim3(s.grid), blk, 0, st, d, k);
+}
+
+template <int LAT, int SLOTS>
+void check_by_degree(const SysDesc& d, const BatchLaunch* b, uint32_t k, const LaunchShape& s, hipStream_t st) {
+    const uint32_t deg = b ? b->maxDegree : d.maxDegree;
+    if (deg <= 2)
+        check_launch<LAT, SLOTS, 2>(d, b, k, s, st);
+    else if (deg <= 4)
+        check_launch<LAT, SLOTS, 4>(d, b, k, s, st);
+    else
+        check_launch<LAT, SLOTS, 8>(d, b, k, s, st);
 }
 
 template <int LAT>
-hipError_t check_by_slots(const SysDesc& d, uint32_t k, const LaunchShape& s, hipStream_t st) {
+void check_by_slots(const SysDesc& d, const BatchLaunch* b, uint32_t k, const LaunchShape& s, hipStream_t st) {
     switch (s.slots) {
-        case 1: return check_by_degree<LAT, 1>(d, k, s, st);
-        case 2: return check_by_degree<LAT, 2>(d, k, s, st);
-        default: return check_by_degree<LAT, 4>(d, k, s, st);
+        case 1: check_by_degree<LAT, 1>(d, b, k, s, st); break;
+        case 2: check_by_degree<LAT, 2>(d, b, k, s, st); break;
+        default: check_by_degree<LAT, 4>(d, b, k, s, st); break;
     }
 }
 
+template <int LAT, int SLOTS>
+void perform_launch(const SysDesc& d, const BatchLaunch* b, uint32_t k, const LaunchShape& s, hipStream_t st) {
+    const dim3 blk(s.block);
+    if (b)
+        hipLaunchKernelGGL((k_perform_batch<LAT, SLOTS>), dim3(b->tiles), blk, 0, st, b->descs, b->tileTable,
+                           b->lists, b->listCounts, k);
+    else
+        hipLaunchKernelGGL((k_perform<LAT, SLOTS>), dim3(s.grid), blk, 0, st, d, k);
+}
+
 template <int LAT>
-hipError_t perform_by_slots(const SysDesc& d, uint32_t k, const LaunchShape& s, hipStream_t st) {
-    const dim3 g(s.grid), b(s.block);
+void perform_by_slots(const SysDesc& d, const BatchLaunch* b, uint32_t k, const LaunchShape& s, hipStream_t st) {
     switch (s.slots) {
-        case 1: hipLaunchKernelGGL((k_perform<LAT, 1>), g, b, 0, st, d, k); break;
-        case 2: hipLaunchKernelGGL((k_perform<LAT, 2>), g, b, 0, st, d, k); break;
-        default: hipLaunchKernelGGL((k_perform<LAT, 4>), g, b, 0, st, d, k); break;
+        case 1: perform_launch<LAT, 1>(d, b, k, s, st); break;
+        case 2: perform_launch<LAT, 2>(d, b, k, s, st); break;
+        default: perform_launch<LAT, 4>(d, b, k, s, st); break;
     }
-    return hipGetLastError();
 }
 
 }  // namespace
 
 hipError_t launch_check(const SysDesc& d, uint32_t k, const LaunchShape& s, hipStream_t st) {
-    return d.latticeKind == kLatticeU8 ? check_by_slots<kLatticeU8>(d, k, s, st)
-                                       : check_by_slots<kLatticeBit>(d, k, s, st);
+    if (d.latticeKind == kLatticeU8)
+        check_by_slots<kLatticeU8>(d, nullptr, k, s, st);
+    else
+        check_by_slots<kLatticeBit>(d, nullptr, k, s, st);
+    return hipGetLastError();
 }
 
 hipError_t launch_perform(const SysDesc& d, uint32_t k, const LaunchShape& s, hipStream_t st) {
-    return d.latticeKind == kLatticeU8 ? perform_by_slots<kLatticeU8>(d, k, s, st)
-                                       : perform_by_slots<kLatticeBit>(d, k, s, st);
+    if (d.latticeKind == kLatticeU8)
+        perform_by_slots<kLatticeU8>(d, nullptr, k, s, st);
+    else
+        perform_by_slots<kLatticeBit>(d, nullptr, k, s, st);
+    return hipGetLastError();
+}
+
+hipError_t launch_check_batch(const BatchLaunch& b, uint32_t k, const LaunchShape& s, hipStream_t st) {
+    const SysDesc none{};
+    if (b.latticeKind == kLatticeU8)
+        check_by_slots<kLatticeU8>(none, &b, k, s, st);
+    else
+        check_by_slots<kLatticeBit>(none, &b, k, s, st);
+    return hipGetLastError();
+}
+
+hipError_t launch_perform_batch(const BatchLaunch& b, uint32_t k, const LaunchShape& s, hipStream_t st) {
+    const SysDesc none{};
+    if (b.latticeKind == kLatticeU8)
+        perform_by_slots<kLatticeU8>(none, &b, k, s, st);
+    else
+        perform_by_slots<kLatticeBit>(none, &b, k, s, st);
+    return hipGetLastError();
 }
 
 hipError_t launch_advance(DevState* state, uint32_t count, hipStream_t st) {
     hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, st, state, count);
+    return hipGetLastError();
+}
+
+hipError_t launch_advance_batch(const SysDesc* descs, uint32_t nDescs, uint32_t count, hipStream_t st) {
+    if (nDescs == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_advance_batch, dim3((nDescs + 255) / 256), dim3(256), 0, st, descs, nDescs, count);
+    return hipGetLastError();
+}
+
+hipError_t launch_set_stops(const SysDesc* descs, uint32_t nDescs, const uint64_t* stops, hipStream_t st) {
+    if (nDescs == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_set_stops, dim3((nDescs + 255) / 256), dim3(256), 0, st, descs, nDescs, stops);
+    return hipGetLastError();
+}
+
+hipError_t launch_gather_state(const SysDesc* descs, uint32_t nDescs, uint64_t* out, hipStream_t st) {
+    if (nDescs == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_gather_state, dim3(nDescs), dim3(256), 0, st, descs, nDescs, out);
     return hipGetLastError();
 }
 
@@ -339,15 +492,6 @@ hipError_t launch_fill_u32(uint32_t* p, uint64_t n, uint32_t v, hipStream_t st) 
     hipLaunchKernelGGL(k_fill_u32, dim3(uint32_t(g)), dim3(256), 0, st, p, n, v);
     return hipGetLastError();
 }
-
-namespace {
-// Plain streaming copy: the reference point that scales the HBM byte
-// counters of a profiler run (it reads and writes exactly n*16 bytes).
-__global__ void k_copy_u4(const uint4* __restrict__ src, uint4* __restrict__ dst, uint64_t n) {
-    for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += uint64_t(gridDim.x) * blockDim.x)
-        dst[i] = src[i];
-}
-}  // namespace
 
 hipError_t launch_copy_u4(const void* src, void* dst, uint64_t n16, hipStream_t st) {
     if (n16 == 0) return hipSuccess;

@@ -12,19 +12,7 @@ namespace pscbfm {
 
 namespace {
 
-[[noreturn]] void hip_fail(hipError_t e, const char* what) {
-    std::ostringstream msg;
-    msg << "HIP error in " << what << ": " << hipGetErrorName(e) << " (" << hipGetErrorString(e) << ")";
-    throw std::runtime_error(msg.str());
-}
-
-inline void hip_check(hipError_t e, const char* what) {
-    if (e != hipSuccess) hip_fail(e, what);
-}
-
 bool is_pow2(uint32_t v) { return v != 0 && (v & (v - 1)) == 0; }
-
-uint64_t align_up(uint64_t v, uint64_t a) { return (v + a - 1) / a * a; }
 
 }  // namespace
 
@@ -281,36 +269,17 @@ void Updater::freeDevice() {
         (void)hipGraphExecDestroy(graphExec_);
         graphExec_ = nullptr;
     }
-    if (arena_) {
-        (void)hipFree(arena_);
-        arena_ = nullptr;
-    }
+    dev_.release();
     if (stream_) {
         (void)hipStreamDestroy(stream_);
         stream_ = nullptr;
     }
-    arenaBytes_ = 0;
 }
 
 // UpdaterGPUScBFM_AB_Type.cu:490-690
-void Updater::initialize() {
-    if (initialized_) {
-        throw std::runtime_error(
-            "[UpdaterGPUScBFM_AB_Type::initialize] Initialize was already called and may not be called again until "
-            "cleanup was called!");
-    }
+SystemInput Updater::stagedInput() const {
     requireMonomers("initialize");
     if (box_[0] == 0) throw std::runtime_error("[UpdaterGPUScBFM_AB_Type::initialize] setLatticeSize first");
-    unsigned nAllowed = 0;
-    for (int i = 0; i < 512; ++i) nAllowed += forbidden_[i] ? 0u : 1u;
-    if (nAllowed != 108) {
-        std::ostringstream msg;
-        msg << "[UpdaterGPUScBFM_AB_Type::initialize] Wrong bond-set! Expected 108 allowed bonds, but got "
-            << nAllowed;
-        throw std::runtime_error(msg.str());
-    }
-    if (checkOnInit_) checkSystem();
-
     SystemInput in;
     std::memcpy(in.box, box_, sizeof box_);
     in.xyz = xyz_;
@@ -321,7 +290,32 @@ void Updater::initialize() {
         for (uint32_t k = 0; k < degree_[i]; ++k)
             in.bonds.ids[in.bonds.start[i] + k] = nbrIds_[size_t(i) * kMaxConnectivity + k];
     std::memcpy(in.forbidden, forbidden_, sizeof forbidden_);
-    layout_ = build_layout(in, hasColorOverride_ ? &colorOverride_ : nullptr, uniformColors_);
+    return in;
+}
+
+// Checks of UpdaterGPUScBFM_AB_Type.cu:514-530 (108 allowed bonds) and the
+// optional invariant check, then the species layout.
+SystemLayout Updater::buildLayout() const {
+    const SystemInput in = stagedInput();
+    unsigned nAllowed = 0;
+    for (int i = 0; i < 512; ++i) nAllowed += forbidden_[i] ? 0u : 1u;
+    if (nAllowed != 108) {
+        std::ostringstream msg;
+        msg << "[UpdaterGPUScBFM_AB_Type::initialize] Wrong bond-set! Expected 108 allowed bonds, but got "
+            << nAllowed;
+        throw std::runtime_error(msg.str());
+    }
+    if (checkOnInit_) checkSystem();
+    return build_layout(in, hasColorOverride_ ? &colorOverride_ : nullptr, uniformColors_);
+}
+
+void Updater::initialize() {
+    if (initialized_) {
+        throw std::runtime_error(
+            "[UpdaterGPUScBFM_AB_Type::initialize] Initialize was already called and may not be called again until "
+            "cleanup was called!");
+    }
+    layout_ = buildLayout();
 
     if (gpu_ < 0) {
         int dev = 0;
@@ -333,95 +327,8 @@ void Updater::initialize() {
     hip_check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
 
     cells_ = uint64_t(box_[0]) * box_[1] * box_[2];
-    const uint64_t latWords = cells_ / 32;
-    const uint64_t nSlots = std::max<uint64_t>(layout_.nSlots, 1);
-    struct Piece {
-        uint64_t bytes;
-        uint64_t off;
-    };
-    enum { P_POS, P_IMGX, P_IMGY, P_IMGZ, P_LIST, P_LCOUNT, P_NBR, P_LAT, P_PROP, P_COLORS, P_BOND, P_STATE,
-           P_ACC, P_N };
-    const uint64_t gridCap = max_grid(layout_.maxColorCount);
-    Piece p[P_N];
-    p[P_POS].bytes = nSlots * 4;
-    p[P_IMGX].bytes = nSlots * 4;
-    p[P_IMGY].bytes = nSlots * 4;
-    p[P_IMGZ].bytes = nSlots * 4;
-    p[P_LIST].bytes = list_capacity(layout_.maxColorCount) * 8;
-    p[P_LCOUNT].bytes = gridCap * 4;
-    p[P_NBR].bytes = std::max<uint64_t>(layout_.nbr.size(), 1) * 4;
-    p[P_LAT].bytes = latticeKind_ == kLatticeU8 ? cells_ : latWords * 4;
-    p[P_PROP].bytes = 2 * latWords * 4;
-    p[P_COLORS].bytes = std::max<uint64_t>(layout_.colors.size(), 1) * sizeof(ColorInfo);
-    p[P_BOND].bytes = 64;
-    p[P_STATE].bytes = sizeof(DevState);
-    p[P_ACC].bytes = gridCap * sizeof(uint64_t);
-    uint64_t total = 0;
-    for (int i = 0; i < P_N; ++i) {
-        p[i].off = total;
-        total = align_up(total + p[i].bytes, 256);
-    }
-    hip_check(hipMalloc(&arena_, total), "hipMalloc");
-    arenaBytes_ = total;
-    char* base = static_cast<char*>(arena_);
-
-    desc_ = SysDesc{};
-    desc_.pos = reinterpret_cast<uint32_t*>(base + p[P_POS].off);
-    desc_.imgX = reinterpret_cast<int32_t*>(base + p[P_IMGX].off);
-    desc_.imgY = reinterpret_cast<int32_t*>(base + p[P_IMGY].off);
-    desc_.imgZ = reinterpret_cast<int32_t*>(base + p[P_IMGZ].off);
-    desc_.list = reinterpret_cast<uint2*>(base + p[P_LIST].off);
-    desc_.listCount = reinterpret_cast<uint32_t*>(base + p[P_LCOUNT].off);
-    desc_.nbr = reinterpret_cast<const uint32_t*>(base + p[P_NBR].off);
-    desc_.lattice = base + p[P_LAT].off;
-    desc_.proposal = reinterpret_cast<uint32_t*>(base + p[P_PROP].off);
-    desc_.colors = reinterpret_cast<const ColorInfo*>(base + p[P_COLORS].off);
-    desc_.bondMask = reinterpret_cast<const uint32_t*>(base + p[P_BOND].off);
-    desc_.state = reinterpret_cast<DevState*>(base + p[P_STATE].off);
-    desc_.seed = seed_;
-    for (int a = 0; a < 3; ++a) desc_.mask[a] = layout_.codec.mask[a];
-    desc_.nColors = uint32_t(layout_.colors.size());
-    desc_.latWords = uint32_t(latWords);
-    desc_.maxDegree = 0;
-    for (const ColorInfo& ci : layout_.colors) desc_.maxDegree = std::max(desc_.maxDegree, ci.maxDegree);
-    desc_.latticeKind = latticeKind_;
-    desc_.maxColorCount = layout_.maxColorCount;
-
-    // Host images of every device buffer, uploaded once.
-    hip_check(hipMemsetAsync(arena_, 0, total, stream_), "hipMemsetAsync");
-    auto up = [&](int piece, const void* src, uint64_t bytes) {
-        if (bytes) hip_check(hipMemcpyAsync(base + p[piece].off, src, bytes, hipMemcpyHostToDevice, stream_),
-                             "hipMemcpyAsync H2D");
-    };
-    up(P_POS, layout_.pos.data(), uint64_t(layout_.nSlots) * 4);
-    up(P_IMGX, layout_.img[0].data(), uint64_t(layout_.nSlots) * 4);
-    up(P_IMGY, layout_.img[1].data(), uint64_t(layout_.nSlots) * 4);
-    up(P_IMGZ, layout_.img[2].data(), uint64_t(layout_.nSlots) * 4);
-    up(P_NBR, layout_.nbr.data(), uint64_t(layout_.nbr.size()) * 4);
-    up(P_COLORS, layout_.colors.data(), uint64_t(layout_.colors.size()) * sizeof(ColorInfo));
-    up(P_BOND, layout_.bondMask, 64);
-    {
-        // Occupancy lattice in Z-curve order: byte per cell or bit per cell.
-        if (latticeKind_ == kLatticeU8) {
-            std::vector<uint8_t> lat(cells_, 0);
-            for (uint32_t i = 0; i < n_; ++i) lat[layout_.pos[layout_.slotOf[i]]] = 1;
-            up(P_LAT, lat.data(), cells_);
-        } else {
-            std::vector<uint32_t> lat(latWords, 0);
-            for (uint32_t i = 0; i < n_; ++i) {
-                const uint32_t c = layout_.pos[layout_.slotOf[i]];
-                lat[c >> 5] |= 1u << (c & 31u);
-            }
-            up(P_LAT, lat.data(), latWords * 4);
-        }
-    }
-    DevState st{};
-    st.substep = 0;
-    st.log = nullptr;
-    st.stopAt = std::numeric_limits<uint64_t>::max();
-    st.acceptedByBlock = reinterpret_cast<uint64_t*>(base + p[P_ACC].off);
-    up(P_STATE, &st, sizeof st);
-    hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize (initialize)");
+    dev_.allocate(layout_, latticeKind_, seed_, /*withLists=*/true, uint32_t(max_grid(layout_.maxColorCount)),
+                  stream_);
 
     performGridMax_ = 0;
     applyLaunchShape();
@@ -443,10 +350,10 @@ void Updater::captureGraph() {
     hip_check(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
     hipError_t err = hipSuccess;
     for (uint32_t k = 0; k < g && err == hipSuccess; ++k) {
-        err = launch_check(desc_, k, shape_, stream_);
-        if (err == hipSuccess) err = launch_perform(desc_, k, shape_, stream_);
+        err = launch_check(dev_.desc, k, shape_, stream_);
+        if (err == hipSuccess) err = launch_perform(dev_.desc, k, shape_, stream_);
     }
-    if (err == hipSuccess) err = launch_advance(desc_.state, g, stream_);
+    if (err == hipSuccess) err = launch_advance(dev_.desc.state, g, stream_);
     const hipError_t endErr = hipStreamEndCapture(stream_, &graph);
     if (err != hipSuccess) hip_fail(err, "kernel launch during capture");
     hip_check(endErr, "hipStreamEndCapture");
@@ -470,12 +377,12 @@ void Updater::run(int32_t nMCS, uint32_t* log, bool profile, bool useGraph) {
         hip_check(hipMalloc(&dLog, S * 4), "hipMalloc (log)");
         hip_check(launch_fill_u32(dLog, S, 0u, stream_), "fill log");
         DevState patch{};
-        hip_check(hipMemcpyAsync(&patch, desc_.state, sizeof patch, hipMemcpyDeviceToHost, stream_), "state D2H");
+        hip_check(hipMemcpyAsync(&patch, dev_.desc.state, sizeof patch, hipMemcpyDeviceToHost, stream_), "state D2H");
         hip_check(hipStreamSynchronize(stream_), "sync");
         patch.log = dLog;
         patch.logFirst = substep_;
         patch.logCapacity = S;
-        hip_check(hipMemcpyAsync(desc_.state, &patch, sizeof patch, hipMemcpyHostToDevice, stream_), "state H2D");
+        hip_check(hipMemcpyAsync(dev_.desc.state, &patch, sizeof patch, hipMemcpyHostToDevice, stream_), "state H2D");
     }
 
     hipEvent_t e0, e1;
@@ -488,11 +395,11 @@ void Updater::run(int32_t nMCS, uint32_t* log, bool profile, bool useGraph) {
         for (auto& e : kev) hip_check(hipEventCreate(&e), "hipEventCreate");
         for (uint64_t k = 0; k < S; ++k) {
             hip_check(hipEventRecord(kev[3 * k], stream_), "hipEventRecord");
-            hip_check(launch_check(desc_, 0, shape_, stream_), "launch pass 1");
+            hip_check(launch_check(dev_.desc, 0, shape_, stream_), "launch pass 1");
             hip_check(hipEventRecord(kev[3 * k + 1], stream_), "hipEventRecord");
-            hip_check(launch_perform(desc_, 0, shape_, stream_), "launch pass 2");
+            hip_check(launch_perform(dev_.desc, 0, shape_, stream_), "launch pass 2");
             hip_check(hipEventRecord(kev[3 * k + 2], stream_), "hipEventRecord");
-            hip_check(launch_advance(desc_.state, 1, stream_), "launch advance");
+            hip_check(launch_advance(dev_.desc.state, 1, stream_), "launch advance");
         }
     } else {
         uint64_t done = 0;
@@ -509,10 +416,10 @@ void Updater::run(int32_t nMCS, uint32_t* log, bool profile, bool useGraph) {
         while (left > 0) {
             const uint32_t blk = uint32_t(std::min<uint64_t>(left, 1u << 20));
             for (uint32_t k = 0; k < blk; ++k) {
-                hip_check(launch_check(desc_, k, shape_, stream_), "launch pass 1");
-                hip_check(launch_perform(desc_, k, shape_, stream_), "launch pass 2");
+                hip_check(launch_check(dev_.desc, k, shape_, stream_), "launch pass 1");
+                hip_check(launch_perform(dev_.desc, k, shape_, stream_), "launch pass 2");
             }
-            hip_check(launch_advance(desc_.state, blk, stream_), "launch advance");
+            hip_check(launch_advance(dev_.desc.state, blk, stream_), "launch advance");
             left -= blk;
         }
     }
@@ -554,17 +461,17 @@ void Updater::run(int32_t nMCS, uint32_t* log, bool profile, bool useGraph) {
     if (log) {
         hip_check(hipMemcpy(log, dLog, S * 4, hipMemcpyDeviceToHost), "log D2H");
         DevState patch{};
-        hip_check(hipMemcpy(&patch, desc_.state, sizeof patch, hipMemcpyDeviceToHost), "state D2H");
+        hip_check(hipMemcpy(&patch, dev_.desc.state, sizeof patch, hipMemcpyDeviceToHost), "state D2H");
         patch.log = nullptr;
         patch.logCapacity = 0;
-        hip_check(hipMemcpy(desc_.state, &patch, sizeof patch, hipMemcpyHostToDevice), "state H2D");
+        hip_check(hipMemcpy(dev_.desc.state, &patch, sizeof patch, hipMemcpyHostToDevice), "state H2D");
         (void)hipFree(dLog);
     }
     syncFromDevice();
     lastRun_.accepted = 0;
     {
         DevState st{};
-        hip_check(hipMemcpy(&st, desc_.state, sizeof st, hipMemcpyDeviceToHost), "state D2H");
+        hip_check(hipMemcpy(&st, dev_.desc.state, sizeof st, hipMemcpyDeviceToHost), "state D2H");
         std::vector<uint64_t> perBlock(performGridMax_);
         hip_check(hipMemcpy(perBlock.data(), st.acceptedByBlock, perBlock.size() * sizeof(uint64_t),
                             hipMemcpyDeviceToHost),
@@ -580,28 +487,16 @@ void Updater::run(int32_t nMCS, uint32_t* log, bool profile, bool useGraph) {
 // CRITICAL.cu:414-698 (synchronous; returns after copy-back and unsort)
 void Updater::runSimulationOnGPU(int32_t nMCS) { run(nMCS, nullptr, false, true); }
 
-void Updater::syncFromDevice() {
-    const uint32_t ns = layout_.nSlots;
-    std::vector<uint32_t> pos(ns);
-    std::vector<int32_t> img[3];
-    for (auto& v : img) v.resize(ns);
-    if (ns) {
-        hip_check(hipMemcpy(pos.data(), desc_.pos, size_t(ns) * 4, hipMemcpyDeviceToHost), "pos D2H");
-        hip_check(hipMemcpy(img[0].data(), desc_.imgX, size_t(ns) * 4, hipMemcpyDeviceToHost), "img D2H");
-        hip_check(hipMemcpy(img[1].data(), desc_.imgY, size_t(ns) * 4, hipMemcpyDeviceToHost), "img D2H");
-        hip_check(hipMemcpy(img[2].data(), desc_.imgZ, size_t(ns) * 4, hipMemcpyDeviceToHost), "img D2H");
-    }
-    layout_.unfolded(pos, img, xyz_.data());
-}
+void Updater::syncFromDevice() { dev_.positions(layout_, xyz_.data(), stream_); }
 
 void Updater::downloadLattice(std::vector<uint8_t>& out) const {
     if (!initialized_) throw std::runtime_error("[downloadLattice] not initialized");
     out.assign(cells_, 0);
     if (latticeKind_ == kLatticeU8) {
-        hip_check(hipMemcpy(out.data(), desc_.lattice, cells_, hipMemcpyDeviceToHost), "lattice D2H");
+        hip_check(hipMemcpy(out.data(), dev_.desc.lattice, cells_, hipMemcpyDeviceToHost), "lattice D2H");
     } else {
         std::vector<uint32_t> w(cells_ / 32);
-        hip_check(hipMemcpy(w.data(), desc_.lattice, w.size() * 4, hipMemcpyDeviceToHost), "lattice D2H");
+        hip_check(hipMemcpy(w.data(), dev_.desc.lattice, w.size() * 4, hipMemcpyDeviceToHost), "lattice D2H");
         for (uint64_t c = 0; c < cells_; ++c) out[c] = (w[c >> 5] >> (c & 31u)) & 1u;
     }
 }
@@ -609,7 +504,7 @@ void Updater::downloadLattice(std::vector<uint8_t>& out) const {
 void Updater::downloadProposal(std::vector<uint32_t>& out) const {
     if (!initialized_) throw std::runtime_error("[downloadProposal] not initialized");
     out.resize(2 * (cells_ / 32));
-    hip_check(hipMemcpy(out.data(), desc_.proposal, out.size() * 4, hipMemcpyDeviceToHost), "proposal D2H");
+    hip_check(hipMemcpy(out.data(), dev_.desc.proposal, out.size() * 4, hipMemcpyDeviceToHost), "proposal D2H");
 }
 
 int32_t Updater::getMonomerPositionInX(uint32_t i) const {

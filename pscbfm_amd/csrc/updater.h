@@ -10,6 +10,7 @@
 #include <vector>
 
 #include "common.h"
+#include "device_system.h"
 #include "kernels.h"
 #include "layout.h"
 
@@ -77,7 +78,11 @@ public:
     const RunStats& lastRun() const { return lastRun_; }
     uint64_t substepCounter() const { return substep_; }
     uint64_t acceptedTotal() const { return acceptedTotal_; }
-    uint64_t deviceBytes() const { return arenaBytes_; }
+    uint64_t deviceBytes() const { return dev_.bytes; }
+    // Host input as staged by the setters, and the validated species layout
+    // initialize() would build from it (used to hand a system to a batch).
+    SystemInput stagedInput() const;
+    SystemLayout buildLayout() const;
     bool initialized() const { return initialized_; }
     uint32_t latticeKind() const { return latticeKind_; }
     const LaunchShape& launchShape() const { return shape_; }
@@ -124,9 +129,7 @@ private:
     bool initialized_ = false;
     SystemLayout layout_;
     hipStream_t stream_ = nullptr;
-    void* arena_ = nullptr;
-    uint64_t arenaBytes_ = 0;
-    SysDesc desc_{};
+    DeviceSystem dev_;
     LaunchShape shape_;
     hipGraphExec_t graphExec_ = nullptr;
     uint32_t graphSubsteps_ = 0;

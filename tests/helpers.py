@@ -60,9 +60,10 @@ def forbidden_table() -> np.ndarray:
     return oracle.standard_bondset()
 
 
-def make_updater(s: Sys, seed: int, kind: int = P.LATTICE_BIT, colors=None, gpu: int = 0, bulk: bool = True,
-                 block: int = 0, slots: int = 0) -> P.UpdaterGPUScBFM_AB_Type:
-    """Configures an updater in the call order of GPUScBFM_AB_Type.h:83-130."""
+def stage_updater(s: Sys, seed: int, kind: int = P.LATTICE_BIT, colors=None, gpu: int = 0, bulk: bool = True,
+                  block: int = 0, slots: int = 0) -> P.UpdaterGPUScBFM_AB_Type:
+    """Stages a system in the call order of GPUScBFM_AB_Type.h:83-130, up to
+    (not including) initialize."""
     u = P.UpdaterGPUScBFM_AB_Type()
     u.setGpu(gpu)
     u.setPeriodicity(True, True, True)
@@ -88,6 +89,13 @@ def make_updater(s: Sys, seed: int, kind: int = P.LATTICE_BIT, colors=None, gpu:
         u.setColors(colors)
     if block or slots:
         u.setLaunchShape(block, slots)
+    return u
+
+
+def make_updater(s: Sys, seed: int, kind: int = P.LATTICE_BIT, colors=None, gpu: int = 0, bulk: bool = True,
+                 block: int = 0, slots: int = 0) -> P.UpdaterGPUScBFM_AB_Type:
+    """Configures and initializes an updater (GPUScBFM_AB_Type.h:83-130)."""
+    u = stage_updater(s, seed, kind, colors, gpu, bulk, block, slots)
     u.initialize()
     return u
 
