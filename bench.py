@@ -33,6 +33,7 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 import pscbfm_amd as P  # noqa: E402
+from pscbfm_amd import workloads as W  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
 METRIC = "attempted monomer moves/sec (1 GPU) + % HBM roofline; batched vs plain at bit-exact"
@@ -42,41 +43,9 @@ def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def undirected_to_directed(n: int, bonds: np.ndarray) -> np.ndarray:
-    a = np.concatenate([bonds[:, 0], bonds[:, 1]]).astype(np.int64)
-    b = np.concatenate([bonds[:, 1], bonds[:, 0]]).astype(np.int64)
-    order = np.argsort(a, kind="stable")
-    return np.stack([a[order], b[order]], axis=1).astype(np.uint32)
-
-
-def standard_forbidden() -> np.ndarray:
-    forb = np.ones(512, dtype=np.uint8)
-    for v in P.standard_bond_vectors():
-        dx, dy, dz = (int(c) for c in v)
-        forb[(dx & 7) | ((dy & 7) << 3) | ((dz & 7) << 6)] = 0
-    return forb
-
-
 def make_updater(box, xyz, bonds, seed, kind, block=0, slots=0):
-    u = P.UpdaterGPUScBFM_AB_Type()
-    u.setGpu(0)
-    u.setPeriodicity(True, True, True)
-    u.setNrOfAllMonomers(xyz.shape[0])
-    u.setMonomerCoordinatesBulk(xyz)
-    u.setConnectivityBulk(undirected_to_directed(xyz.shape[0], bonds))
-    u.setLatticeSize(*box)
-    forb = standard_forbidden()
-    for dx in range(-4, 4):
-        for dy in range(-4, 4):
-            for dz in range(-4, 4):
-                u.copyBondSet(dx, dy, dz, bool(forb[(dx & 7) | ((dy & 7) << 3) | ((dz & 7) << 6)]))
-    u.setSeed(seed)
-    u.setLatticeKind(kind)
-    u.setCheckOnInitialize(False)
-    if block or slots:
-        u.setLaunchShape(block, slots)
-    u.initialize()
-    return u, forb
+    sd = W.SystemData(tuple(box), xyz, bonds, seed, "ordered")
+    return W.stage(sd, kind, block=block, slots=slots, initialize=True)
 
 
 def algorithmic_bytes(stats, lattice_bytes_per_cell: float):
@@ -133,6 +102,79 @@ def cpu_baseline(u, box, bonds, colors, seed, target_seconds: float):
     }, parity, mcs
 
 
+def batched_vs_plain(n_rep: int, mcs: int, seed: int, kind: int, tune_sample: int, tune_mcs: int) -> dict:
+    """Replica mix: one batch (autotuned shape, graph-replayed launches over
+    every replica) against the plain loop of one updater per system, each
+    called with runSimulationOnGPU(mcs) in turn.  Every replica's final
+    coordinates are compared bit for bit between the two."""
+    tic = time.perf_counter()
+    specs = W.replica_specs(n_rep, seed)
+    reps = W.build_replicas(specs)
+    n_total = sum(r.n for r in reps)
+    n_ordered = sum(r.method == "ordered" for r in reps)
+    staged = [W.stage(r, kind) for r in reps]
+    log(f"replica mix: {n_rep} systems, {n_total} monomers ({n_ordered} by ordered fallback), "
+        f"built and staged in {time.perf_counter() - tic:.1f} s")
+
+    rng = np.random.default_rng(seed + 1)
+    pick = sorted(rng.choice(n_rep, size=min(tune_sample, n_rep), replace=False).tolist())
+    tic = time.perf_counter()
+    block, slots, trials = P.autotune_batch([staged[i] for i in pick], tune_mcs, kind)
+    log(f"autotune on {len(pick)} replicas: block {block} slots {slots} ({time.perf_counter() - tic:.1f} s); "
+        + ", ".join(f"{b}x{s_}:{ms:.3f}" for (b, s_), ms in sorted(trials.items())))
+
+    batch = P.Batch(0, n_rep, kind, block, slots)
+    tic = time.perf_counter()
+    ids = [batch.add(u) for u in staged]
+    live, used_tiles, cap, dev_bytes = batch.usage()
+    log(f"batch: {live} replicas, {used_tiles} workgroups, {dev_bytes / 2**20:.0f} MiB "
+        f"(added in {time.perf_counter() - tic:.1f} s)")
+    tic = time.perf_counter()
+    batch_ms = batch.run(mcs)
+    b_xyz = [batch.positions(i, r.n) for i, r in zip(ids, reps)]
+    batch_wall = time.perf_counter() - tic
+
+    for u in staged:
+        u.initialize()
+    tic = time.perf_counter()
+    plain_ms = 0.0
+    attempted = 0
+    for u in staged:
+        u.runSimulationOnGPU(mcs)
+        st = u.lastRunStats()
+        plain_ms += st.gpu_ms
+        attempted += st.attempted
+    plain_wall = time.perf_counter() - tic
+    same = sum(bool(np.array_equal(u.getPositions(r.n), x)) for u, r, x in zip(staged, reps, b_xyz))
+    accepted_same = sum(int(batch.info(i).accepted == u.counters()[1]) for i, u in zip(ids, staged))
+    for u in staged:
+        u.cleanup()
+    batched = attempted / batch_wall
+    plain = attempted / plain_wall
+    log(f"batched {mcs} MCS: {batch_ms:.1f} ms GPU, {batch_wall * 1e3:.1f} ms wall incl. read-back "
+        f"-> {batched / 1e9:.2f} G attempts/s")
+    log(f"plain loop: {plain_ms:.1f} ms GPU, {plain_wall * 1e3:.1f} ms wall -> {plain / 1e9:.3f} G attempts/s; "
+        f"speedup {batched / plain:.1f}x; bit-identical {same}/{n_rep}, accepted equal {accepted_same}/{n_rep}")
+    return {
+        "workload": f"replica mix: {n_rep} systems, box edges log-uniform in {{32,64,128}} per axis, "
+                    f"chains N ~ U[16,512], phi ~ U[0.1,0.5], SAW growth (ordered fallback for {n_ordered}), "
+                    f"{n_total} monomers, {mcs} MCS each, per-replica seeds from {seed}",
+        "value": batched,
+        "unit": "attempted monomer moves/s (wall, incl. coordinate read-back)",
+        "gpu_value": attempted / (batch_ms * 1e-3),
+        "plain_value": plain,
+        "plain_gpu_value": attempted / (plain_ms * 1e-3),
+        "speedup_vs_plain": batched / plain,
+        "batch_ms": batch_ms,
+        "plain_ms_wall": plain_wall * 1e3,
+        "shape": {"block": block, "slots": slots, "workgroups": used_tiles},
+        "autotune_ms_per_mcs": {f"{b}x{s_}": ms for (b, s_), ms in sorted(trials.items())},
+        "parity": f"{same}/{n_rep} replicas bit-identical to their plain single-system runs "
+                  f"(accepted counts equal for {accepted_same}/{n_rep})",
+        "device_bytes": dev_bytes,
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -148,6 +190,11 @@ def main() -> int:
     ap.add_argument("--profile-mcs", type=int, default=20)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--replicas", type=int, default=1024, help="systems of the batched comparison")
+    ap.add_argument("--replica-mcs", type=int, default=500)
+    ap.add_argument("--tune-sample", type=int, default=64)
+    ap.add_argument("--tune-mcs", type=int, default=5)
+    ap.add_argument("--no-batch", action="store_true")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -166,7 +213,7 @@ def main() -> int:
     tic = time.perf_counter()
     xyz = P.gen_linear_chains_ordered(box, n_chains, args.chain)
     bonds = P.chain_bonds(n_chains, args.chain)
-    u, _ = make_updater(box, xyz, bonds, args.seed, kind, args.block, args.slots)
+    u = make_updater(box, xyz, bonds, args.seed, kind, args.block, args.slots)
     colors = u.colors(xyz.shape[0])
     log(f"initialize done in {time.perf_counter() - tic:.1f} s; species sizes {u.colorSizes().tolist()}; "
         f"device bytes {u.deviceBytes() / 2**20:.0f} MiB")
@@ -258,6 +305,9 @@ def main() -> int:
         out["parity_vs_oracle"] = f"{'bit-exact' if parity else 'MISMATCH'} over {mcs} MCS"
         log(f"CPU {cpu['value'] / 1e6:.1f} M attempts/s on {cpu['cores']} threads; parity {parity}")
     u.cleanup()
+    if not args.no_batch and args.replicas > 0:
+        out["batched"] = batched_vs_plain(args.replicas, args.replica_mcs, args.seed, kind, args.tune_sample,
+                                          args.tune_mcs)
     print(json.dumps(out), flush=True)
     return 0
 

@@ -107,6 +107,15 @@ int pscbfm_set_colors(pscbfm_updater* h, const uint8_t* colors, uint32_t n);
 int pscbfm_set_launch_shape(pscbfm_updater* h, uint32_t block, uint32_t slots);
 /* Substeps per captured graph (0 = automatic). */
 int pscbfm_set_graph_chunk(pscbfm_updater* h, uint32_t substeps);
+/* Launch-shape autotuner.  Counterpart of the reference's in-run thread-count
+ * search (CRITICAL.cu:445-486,600-648), which is compiled out unless
+ * AUTO_CONFIGURE_BEST_SETTINGS_FOR_PSCBFM_ALGORITHM is defined (then fixed at
+ * 128 threads) and which tunes on the live trajectory.  After initialize: times `mcs`
+ * sweeps of a scratch copy of the system for each of the 12 shapes
+ * (block 64/128/192/256 x slots 1/2/4) and applies the fastest; the system's
+ * own state is not touched.  Cached per workload signature in the process.
+ * trial_ms (optional, 12 entries) receives ms per sweep in candidate order. */
+int pscbfm_autotune(pscbfm_updater* h, uint32_t mcs, uint32_t* block, uint32_t* slots, double* trial_ms);
 
 /* Statistics of the last run. */
 typedef struct pscbfm_run_stats {
@@ -203,7 +212,8 @@ typedef struct {
 } pscbfm_replica_info;
 
 /* tile_capacity: workgroups per launch (rounded up to a multiple of 8), the
- * sum of the replicas' tiles may not exceed it; replica_capacity: the most
+ * sum of the replicas' tiles may not exceed it; 0 = a table that grows as
+ * replicas are added (the captured graph is then rebuilt); replica_capacity: the most
  * replicas at once; block (64..256, multiple of 64) and slots (1, 2, 4) as in
  * pscbfm_set_launch_shape; lattice_kind: PSCBFM_LATTICE_*. */
 int pscbfm_batch_create(int gpu, int lattice_kind, uint32_t tile_capacity, uint32_t replica_capacity,
@@ -229,6 +239,11 @@ int pscbfm_batch_run(pscbfm_batch* b, uint64_t n_mcs, double* gpu_ms);
 int pscbfm_batch_info(pscbfm_batch* b, uint32_t id, pscbfm_replica_info* info);
 /* Unfolded coordinates of replica `id` in its input order (3*n ints). */
 int pscbfm_batch_get_positions(pscbfm_batch* b, uint32_t id, int32_t* xyz, uint32_t n);
+/* Autotuner for a batch: times `mcs` sweeps of a scratch batch of the staged
+ * systems for each candidate shape; returns the fastest (to pass to
+ * pscbfm_batch_create) and, optionally, ms per sweep of all 12 candidates. */
+int pscbfm_batch_autotune(int gpu, int lattice_kind, const pscbfm_updater* const* staged, uint32_t n,
+                          uint32_t mcs, uint32_t* block, uint32_t* slots, double* trial_ms);
 int pscbfm_batch_usage(pscbfm_batch* b, uint32_t* live, uint32_t* used_tiles, uint32_t* tile_capacity,
                        uint64_t* device_bytes);
 

@@ -25,6 +25,7 @@ __all__ = [
     "LATTICE_U8", "LATTICE_BIT", "standard_bond_vectors", "move_hash", "substep_draw",
     "zcurve_encode", "color_graph", "gen_linear_chains", "gen_linear_chains_ordered",
     "gen_star_network", "chain_bonds", "Batch", "ReplicaInfo",
+    "autotune_batch", "TUNE_CANDIDATES",
 ]
 
 LATTICE_U8 = 0
@@ -135,6 +136,9 @@ _SIGNATURES = {
     "pscbfm_batch_info": (C.c_int, [_H, C.c_uint32, C.POINTER(ReplicaInfo)]),
     "pscbfm_batch_get_positions": (C.c_int, [_H, C.c_uint32, _i32p, C.c_uint32]),
     "pscbfm_batch_usage": (C.c_int, [_H, _u32p, _u32p, _u32p, _u64p]),
+    "pscbfm_autotune": (C.c_int, [_H, C.c_uint32, _u32p, _u32p, C.POINTER(C.c_double)]),
+    "pscbfm_batch_autotune": (C.c_int, [C.c_int, C.c_int, C.POINTER(_H), C.c_uint32, C.c_uint32, _u32p, _u32p,
+                                        C.POINTER(C.c_double)]),
     "pscbfm_version": (C.c_char_p, []),
 }
 
@@ -301,6 +305,14 @@ class UpdaterGPUScBFM_AB_Type:
                    int(bool(profile)), int(bool(use_graph)), C.byref(stats))
         return stats, buf
 
+    def autotune(self, mcs: int = 20):
+        """Picks and applies the fastest launch shape (scratch copies only);
+        returns (block, slots, {(block, slots): ms per sweep})."""
+        b, sl = C.c_uint32(), C.c_uint32()
+        ms = (C.c_double * len(TUNE_CANDIDATES))()
+        self._call("pscbfm_autotune", int(mcs), C.byref(b), C.byref(sl), ms)
+        return b.value, sl.value, {c: ms[i] for i, c in enumerate(TUNE_CANDIDATES) if ms[i] >= 0}
+
     def lastRunStats(self) -> RunStats:
         s = RunStats()
         self._call("pscbfm_last_run_stats", C.byref(s))
@@ -343,14 +355,30 @@ class UpdaterGPUScBFM_AB_Type:
         return a.value, b.value
 
 
+TUNE_CANDIDATES = [(b, s) for b in (64, 128, 192, 256) for s in (1, 2, 4)]
+
+
+def autotune_batch(staged, mcs: int = 5, lattice_kind: int = LATTICE_BIT, gpu: int = 0):
+    """Fastest (block, slots) for a batch of the staged systems (a sample of
+    the real mix); returns (block, slots, {(block, slots): ms per sweep})."""
+    arr = (C.c_void_p * len(staged))(*[u._h.value for u in staged])
+    b, sl = C.c_uint32(), C.c_uint32()
+    ms = (C.c_double * len(TUNE_CANDIDATES))()
+    code = lib().pscbfm_batch_autotune(int(gpu), int(lattice_kind), arr, len(staged), int(mcs), C.byref(b),
+                                       C.byref(sl), ms)
+    _raise(code, (lib().pscbfm_create_error() or b"").decode())
+    return b.value, sl.value, {c: ms[i] for i, c in enumerate(TUNE_CANDIDATES) if ms[i] >= 0}
+
+
 class Batch:
     """Many independent systems swept by one launch per pass (include/pscbfm.h
     pscbfm_batch_*).  A replica is the system staged in an updater (no
     initialize needed) with that updater's seed; it evolves bit-identically to
     the same updater run alone.  Replicas join and leave between launches."""
 
-    def __init__(self, tile_capacity: int, replica_capacity: int, lattice_kind: int = LATTICE_BIT,
+    def __init__(self, tile_capacity: int = 0, replica_capacity: int = 1024, lattice_kind: int = LATTICE_BIT,
                  block: int = 256, slots: int = 2, gpu: int = 0) -> None:
+        """tile_capacity 0: the workgroup table grows with the replicas."""
         self._lib = lib()
         h = C.c_void_p()
         code = self._lib.pscbfm_batch_create(int(gpu), int(lattice_kind), int(tile_capacity),

@@ -27,8 +27,9 @@ Batch::Batch(int gpu, uint32_t latticeKind, uint32_t tileCapacity, uint32_t repl
     if (latticeKind != kLatticeU8 && latticeKind != kLatticeBit) throw std::invalid_argument("unknown lattice kind");
     if (block == 0 || block % 64 != 0 || block > 256) throw std::invalid_argument("block must be 64..256, multiple of 64");
     if (slots != 1 && slots != 2 && slots != 4) throw std::invalid_argument("slots must be 1, 2 or 4");
-    if (tileCapacity == 0 || replicaCapacity == 0) throw std::invalid_argument("capacities must be positive");
-    tileCap_ = (tileCapacity + 7) / 8 * 8;
+    if (replicaCapacity == 0) throw std::invalid_argument("replica capacity must be positive");
+    growable_ = tileCapacity == 0;
+    tileCap_ = growable_ ? 8u : (tileCapacity + 7) / 8 * 8;
     replicaCap_ = replicaCapacity;
     shape_.block = block;
     shape_.slots = slots;
@@ -36,18 +37,43 @@ Batch::Batch(int gpu, uint32_t latticeKind, uint32_t tileCapacity, uint32_t repl
     hip_check(hipSetDevice(gpu_), "hipSetDevice");
     hip_check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
     hip_check(hipMalloc(&dDescs_, sizeof(SysDesc) * replicaCap_), "hipMalloc descs");
-    hip_check(hipMalloc(&dTiles_, sizeof(Tile) * tileCap_), "hipMalloc tiles");
-    hip_check(hipMalloc(&dLists_, sizeof(uint2) * uint64_t(tileCap_) * block * slots), "hipMalloc lists");
-    hip_check(hipMalloc(&dListCounts_, sizeof(uint32_t) * tileCap_), "hipMalloc list counts");
     hip_check(hipMalloc(&dStops_, sizeof(uint64_t) * replicaCap_), "hipMalloc stops");
     hip_check(hipMalloc(&dGather_, sizeof(uint64_t) * 3 * replicaCap_), "hipMalloc gather");
-    hip_check(hipMemsetAsync(dListCounts_, 0, sizeof(uint32_t) * tileCap_, stream_), "memset");
     descs_.assign(replicaCap_, SysDesc{});
-    Tile idle{kNoReplica, 0, 0, 0};
-    tiles_.assign(tileCap_, idle);
     replicas_.resize(replicaCap_);
     freeSlots_.assign(8, {});
-    for (uint32_t w = tileCap_; w-- > 0;) freeSlots_[w % 8].push_back(w);  // pop_back gives the lowest slot
+    const uint32_t cap = tileCap_;
+    tileCap_ = 0;
+    growTiles(cap);
+}
+
+// Tile table, lists and list counts for `needed` workgroups; the new slots
+// are free, the old ones keep their replicas.
+void Batch::growTiles(uint32_t needed) {
+    needed = (needed + 7) / 8 * 8;
+    if (needed <= tileCap_) return;
+    if (stream_) hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+    Tile* tiles = nullptr;
+    uint2* lists = nullptr;
+    uint32_t* counts = nullptr;
+    hip_check(hipMalloc(&tiles, sizeof(Tile) * needed), "hipMalloc tiles");
+    hip_check(hipMalloc(&lists, sizeof(uint2) * uint64_t(needed) * shape_.block * shape_.slots), "hipMalloc lists");
+    hip_check(hipMalloc(&counts, sizeof(uint32_t) * needed), "hipMalloc list counts");
+    if (dTiles_) (void)hipFree(dTiles_);
+    if (dLists_) (void)hipFree(dLists_);
+    if (dListCounts_) (void)hipFree(dListCounts_);
+    dTiles_ = tiles;
+    dLists_ = lists;
+    dListCounts_ = counts;
+    tiles_.resize(needed, Tile{kNoReplica, 0, 0, 0});
+    for (uint32_t w = needed; w-- > tileCap_;) freeSlots_[w % 8].push_back(w);
+    for (auto& f : freeSlots_) std::sort(f.begin(), f.end(), std::greater<uint32_t>());  // pop_back = lowest
+    tileCap_ = needed;
+    shape_.grid = tileCap_;
+    if (graph_) {
+        (void)hipGraphExecDestroy(graph_);
+        graph_ = nullptr;
+    }
     tablesDirty_ = true;
 }
 
@@ -74,6 +100,7 @@ uint32_t Batch::add(const SystemLayout& L, uint64_t seed) {
     while (id < replicaCap_ && replicas_[id].live) ++id;
     if (id == replicaCap_) throw std::runtime_error("batch: replica capacity exhausted");
     const uint32_t nT = tilesNeeded(L);
+    if (growable_ && usedTiles_ + nT > tileCap_) growTiles(std::max(usedTiles_ + nT, tileCap_ + tileCap_ / 2));
     if (usedTiles_ + nT > tileCap_) {
         std::ostringstream msg;
         msg << "batch: tile capacity exhausted (" << usedTiles_ << " + " << nT << " > " << tileCap_ << ")";

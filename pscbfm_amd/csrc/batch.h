@@ -30,8 +30,10 @@ struct BatchReplicaInfo {
 
 class Batch {
 public:
-    // tileCapacity: workgroups per launch (rounded up to a multiple of 8);
-    // replicaCapacity: entries of the descriptor table.
+    // tileCapacity: workgroups per launch (rounded up to a multiple of 8), or
+    // 0 for a table that starts at 8 and grows as replicas are added (a fixed
+    // table rejects a replica that does not fit); replicaCapacity: entries of
+    // the descriptor table.
     Batch(int gpu, uint32_t latticeKind, uint32_t tileCapacity, uint32_t replicaCapacity, uint32_t block = 256,
           uint32_t slots = 2);
     ~Batch();
@@ -77,6 +79,7 @@ private:
     };
     void uploadTables();
     void captureGraph();
+    void growTiles(uint32_t needed);
 
     int gpu_;
     uint32_t latticeKind_;
@@ -96,6 +99,7 @@ private:
     std::vector<std::vector<uint32_t>> freeSlots_;  // per XCD
     std::vector<Replica> replicas_;
     uint32_t usedTiles_ = 0;
+    bool growable_ = false;
     bool tablesDirty_ = true;
     bool stopsDirty_ = false;
     uint32_t graphDegree_ = 0;

@@ -9,6 +9,7 @@
 #include <string>
 #include <vector>
 
+#include "autotune.h"
 #include "batch.h"
 #include "coloring.h"
 #include "common.h"
@@ -436,6 +437,31 @@ int pscbfm_copy_probe(int gpu, uint64_t bytes, int repeats, double* ms_per_copy)
     return rc;
 }
 
+namespace {
+void report_tune(const pscbfm::TuneResult& r, uint32_t* block, uint32_t* slots, double* trialMs) {
+    if (block) *block = r.best.block;
+    if (slots) *slots = r.best.slots;
+    if (trialMs) {
+        const auto& cand = pscbfm::tune_candidates();
+        for (size_t i = 0; i < cand.size(); ++i) {
+            trialMs[i] = -1.0;
+            for (const auto& t : r.trials)
+                if (t.shape.block == cand[i].block && t.shape.slots == cand[i].slots) trialMs[i] = t.msPerMcs;
+        }
+    }
+}
+}  // namespace
+
+int pscbfm_autotune(pscbfm_updater* h, uint32_t mcs, uint32_t* block, uint32_t* slots, double* trial_ms) {
+    return guarded(h, [&](pscbfm::Updater& u) {
+        if (!u.initialized()) throw std::runtime_error("[autotune] initialize has not been called");
+        if (mcs == 0) throw std::invalid_argument("[autotune] mcs must be positive");
+        const pscbfm::TuneResult r = pscbfm::autotune_system(u.gpu(), u.layout(), u.latticeKind(), u.seed(), mcs);
+        u.setLaunchShape(r.best.block, r.best.slots);
+        report_tune(r, block, slots, trial_ms);
+    });
+}
+
 // ---- batch ------------------------------------------------------------------------
 
 int pscbfm_batch_create(int gpu, int lattice_kind, uint32_t tile_capacity, uint32_t replica_capacity,
@@ -517,6 +543,24 @@ int pscbfm_batch_info(pscbfm_batch* b, uint32_t id, pscbfm_replica_info* info) {
 int pscbfm_batch_get_positions(pscbfm_batch* b, uint32_t id, int32_t* xyz, uint32_t n) {
     if (!xyz && n) return PSCBFM_INVALID_ARGUMENT;
     return batch_guarded(b, [&](pscbfm::Batch& e) { e.positions(id, xyz, n); });
+}
+
+int pscbfm_batch_autotune(int gpu, int lattice_kind, const pscbfm_updater* const* staged, uint32_t n,
+                          uint32_t mcs, uint32_t* block, uint32_t* slots, double* trial_ms) {
+    if (!staged || n == 0 || mcs == 0 || gpu < 0) return PSCBFM_INVALID_ARGUMENT;
+    for (uint32_t i = 0; i < n; ++i)
+        if (!staged[i]) return PSCBFM_NULL_HANDLE;
+    return translated(g_createError, [&] {
+        std::vector<pscbfm::SystemLayout> layouts;
+        std::vector<uint64_t> seeds;
+        layouts.reserve(n);
+        for (uint32_t i = 0; i < n; ++i) {
+            layouts.push_back(staged[i]->engine.buildLayout());
+            seeds.push_back(staged[i]->engine.seed());
+        }
+        const pscbfm::TuneResult r = pscbfm::autotune_batch(gpu, layouts, seeds, uint32_t(lattice_kind), mcs);
+        report_tune(r, block, slots, trial_ms);
+    });
 }
 
 int pscbfm_batch_usage(pscbfm_batch* b, uint32_t* live, uint32_t* used_tiles, uint32_t* tile_capacity,

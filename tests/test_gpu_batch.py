@@ -139,3 +139,32 @@ def test_capacity_and_id_errors():
     with pytest.raises(RuntimeError):
         b.add(stage_updater(big, 1))  # tile capacity
     assert b.usage()[0] == 1
+
+
+def test_growable_table_equals_alone():
+    systems = _mix()
+    b = P.Batch(0, len(systems))  # table starts at 8 workgroups and grows
+    ids = [b.add(stage_updater(s, seed)) for s, seed in systems]
+    live, used, cap, _ = b.usage()
+    assert live == len(systems) and used <= cap and cap % 8 == 0
+    b.run(12)
+    for rid, (s, seed) in zip(ids, systems):
+        assert np.array_equal(b.positions(rid, s.n), _alone(s, seed, [12])[0])
+
+
+def test_autotune_leaves_the_trajectory_alone():
+    s, seed = chains((64, 64, 64), 40, 64, seed=11), 5
+    u = make_updater(s, seed)
+    u.run(6)
+    block, slots, trials = u.autotune(3)
+    assert (block, slots) in trials and len(trials) == len(P.TUNE_CANDIDATES)
+    assert min(trials.values()) == trials[(block, slots)]
+    u.run(9)
+    assert np.array_equal(u.getPositions(s.n), _alone(s, seed, [15])[0])
+
+
+def test_batch_autotune_picks_a_candidate():
+    staged = [stage_updater(s, seed) for s, seed in _mix()]
+    block, slots, trials = P.autotune_batch(staged, mcs=2)
+    assert (block, slots) in P.TUNE_CANDIDATES
+    assert trials[(block, slots)] == min(trials.values())
