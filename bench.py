@@ -195,6 +195,9 @@ def main() -> int:
     ap.add_argument("--tune-sample", type=int, default=64)
     ap.add_argument("--tune-mcs", type=int, default=5)
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--serve-rates", default="20,80", help="Poisson arrival rates (jobs/s) of the served stream")
+    ap.add_argument("--serve-seconds", type=float, default=8.0, help="arrival window per rate")
+    ap.add_argument("--no-serve", action="store_true")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -308,6 +311,17 @@ def main() -> int:
     if not args.no_batch and args.replicas > 0:
         out["batched"] = batched_vs_plain(args.replicas, args.replica_mcs, args.seed, kind, args.tune_sample,
                                           args.tune_mcs)
+    if not args.no_serve and args.serve_rates:
+        from pscbfm_amd import serving
+        out["serving"] = {}
+        for k, rate in enumerate(float(r) for r in args.serve_rates.split(",")):
+            jobs = serving.draw_jobs(rate, args.serve_seconds, args.seed + 100 + k)
+            log(f"served stream at {rate:g} jobs/s: {len(jobs)} jobs "
+                f"({sum(j.kind == 'melt' for j in jobs)} dense 512^3 melts)")
+            res = serving.run_stream(jobs, log=log, seed=args.seed)
+            log(f"  {res['jobs_per_s']:.1f} jobs/s, latency p50 {res['latency_p50_s']:.2f} s "
+                f"p99 {res['latency_p99_s']:.2f} s, GPU busy {res['gpu_busy_fraction']:.2f}; {res['parity']}")
+            out["serving"][f"{rate:g}/s"] = res
     print(json.dumps(out), flush=True)
     return 0
 

@@ -247,6 +247,52 @@ int pscbfm_batch_autotune(int gpu, int lattice_kind, const pscbfm_updater* const
 int pscbfm_batch_usage(pscbfm_batch* b, uint32_t* live, uint32_t* used_tiles, uint32_t* tile_capacity,
                        uint64_t* device_bytes);
 
+/* ---- Request loop (continuous batching) -------------------------------------
+ * No counterpart in the reference.  Jobs (a staged updater + a number of
+ * sweeps) are submitted from any thread; a preparation thread builds their
+ * layouts, and a GPU thread admits them into one batch as HBM budget,
+ * workgroups and replica slots allow, launches the live replicas at most
+ * `quantum` substeps at a time (so arrivals join while others run), and
+ * retires finished ones.  Each job's result is the one it has alone. */
+typedef struct pscbfm_server pscbfm_server;
+
+typedef struct {
+    int32_t state;            /* 0 queued, 1 prepared, 2 running, 3 done, 4 failed */
+    uint32_t n_monomers;
+    uint64_t n_mcs;
+    uint64_t accepted;
+    uint64_t device_bytes;    /* footprint while resident */
+    double t_submit, t_admit, t_done;  /* seconds since the server started */
+} pscbfm_job_info;
+
+typedef struct {
+    uint64_t submitted, done, failed;
+    uint32_t queued, running;
+    uint64_t resident_bytes, peak_resident_bytes;
+    uint32_t used_tiles, tile_capacity;
+    uint64_t launches, substeps;
+    double gpu_busy_seconds;
+} pscbfm_server_counters;
+
+/* hbm_budget: device bytes the resident replicas may take together;
+ * tile_capacity (> 0), replica_capacity, block, slots as for pscbfm_batch_create. */
+int pscbfm_server_create(int gpu, int lattice_kind, uint64_t hbm_budget, uint32_t tile_capacity,
+                         uint32_t replica_capacity, uint32_t block, uint32_t slots, uint32_t quantum,
+                         pscbfm_server** out);
+void pscbfm_server_destroy(pscbfm_server* s);
+const char* pscbfm_server_last_error(const pscbfm_server* s);
+/* Queues the system staged in `staged` (its seed included) for n_mcs sweeps. */
+int pscbfm_server_submit(pscbfm_server* s, const pscbfm_updater* staged, uint64_t n_mcs, uint64_t* job);
+/* Waits for the job (timeout_s < 0: forever); *finished = 0 on timeout. */
+int pscbfm_server_wait(pscbfm_server* s, uint64_t job, double timeout_s, int* finished);
+int pscbfm_server_job(pscbfm_server* s, uint64_t job, pscbfm_job_info* info);
+/* Final coordinates of a finished job (3*n ints, input order). */
+int pscbfm_server_result(pscbfm_server* s, uint64_t job, int32_t* xyz, uint32_t n);
+int pscbfm_server_release(pscbfm_server* s, uint64_t job);
+int pscbfm_server_drain(pscbfm_server* s);
+int pscbfm_server_stats(pscbfm_server* s, pscbfm_server_counters* stats);
+int pscbfm_server_now(pscbfm_server* s, double* seconds);
+
 /* Library version string. */
 const char* pscbfm_version(void);
 

@@ -25,7 +25,7 @@ __all__ = [
     "LATTICE_U8", "LATTICE_BIT", "standard_bond_vectors", "move_hash", "substep_draw",
     "zcurve_encode", "color_graph", "gen_linear_chains", "gen_linear_chains_ordered",
     "gen_star_network", "chain_bonds", "Batch", "ReplicaInfo",
-    "autotune_batch", "TUNE_CANDIDATES",
+    "autotune_batch", "TUNE_CANDIDATES", "Server", "JobInfo", "ServerCounters",
 ]
 
 LATTICE_U8 = 0
@@ -59,6 +59,28 @@ class ReplicaInfo(C.Structure):
         ("live", C.c_int32), ("n_colors", C.c_uint32), ("n_monomers", C.c_uint32), ("tiles", C.c_uint32),
         ("substeps", C.c_uint64), ("stop_at", C.c_uint64), ("accepted", C.c_uint64),
         ("device_bytes", C.c_uint64),
+    ]
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+class JobInfo(C.Structure):
+    _fields_ = [
+        ("state", C.c_int32), ("n_monomers", C.c_uint32), ("n_mcs", C.c_uint64), ("accepted", C.c_uint64),
+        ("device_bytes", C.c_uint64), ("t_submit", C.c_double), ("t_admit", C.c_double), ("t_done", C.c_double),
+    ]
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+class ServerCounters(C.Structure):
+    _fields_ = [
+        ("submitted", C.c_uint64), ("done", C.c_uint64), ("failed", C.c_uint64), ("queued", C.c_uint32),
+        ("running", C.c_uint32), ("resident_bytes", C.c_uint64), ("peak_resident_bytes", C.c_uint64),
+        ("used_tiles", C.c_uint32), ("tile_capacity", C.c_uint32), ("launches", C.c_uint64),
+        ("substeps", C.c_uint64), ("gpu_busy_seconds", C.c_double),
     ]
 
     def as_dict(self) -> dict:
@@ -139,6 +161,18 @@ _SIGNATURES = {
     "pscbfm_autotune": (C.c_int, [_H, C.c_uint32, _u32p, _u32p, C.POINTER(C.c_double)]),
     "pscbfm_batch_autotune": (C.c_int, [C.c_int, C.c_int, C.POINTER(_H), C.c_uint32, C.c_uint32, _u32p, _u32p,
                                         C.POINTER(C.c_double)]),
+    "pscbfm_server_create": (C.c_int, [C.c_int, C.c_int, C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32,
+                                        C.c_uint32, C.c_uint32, C.POINTER(_H)]),
+    "pscbfm_server_destroy": (None, [_H]),
+    "pscbfm_server_last_error": (C.c_char_p, [_H]),
+    "pscbfm_server_submit": (C.c_int, [_H, _H, C.c_uint64, _u64p]),
+    "pscbfm_server_wait": (C.c_int, [_H, C.c_uint64, C.c_double, C.POINTER(C.c_int)]),
+    "pscbfm_server_job": (C.c_int, [_H, C.c_uint64, C.POINTER(JobInfo)]),
+    "pscbfm_server_result": (C.c_int, [_H, C.c_uint64, _i32p, C.c_uint32]),
+    "pscbfm_server_release": (C.c_int, [_H, C.c_uint64]),
+    "pscbfm_server_drain": (C.c_int, [_H]),
+    "pscbfm_server_stats": (C.c_int, [_H, C.POINTER(ServerCounters)]),
+    "pscbfm_server_now": (C.c_int, [_H, C.POINTER(C.c_double)]),
     "pscbfm_version": (C.c_char_p, []),
 }
 
@@ -440,6 +474,76 @@ class Batch:
         a, b, c, d = C.c_uint32(), C.c_uint32(), C.c_uint32(), C.c_uint64()
         self._call("pscbfm_batch_usage", C.byref(a), C.byref(b), C.byref(c), C.byref(d))
         return a.value, b.value, c.value, d.value
+
+
+JOB_QUEUED, JOB_PREPARED, JOB_RUNNING, JOB_DONE, JOB_FAILED = range(5)
+
+
+class Server:
+    """Continuous-batching request loop (include/pscbfm.h pscbfm_server_*):
+    submit staged systems with a number of sweeps from any thread; they are
+    admitted into one batch as HBM budget and workgroups allow and run
+    concurrently; each result is the one the system has alone."""
+
+    def __init__(self, hbm_budget: int, tile_capacity: int, replica_capacity: int = 4096,
+                 lattice_kind: int = LATTICE_BIT, block: int = 256, slots: int = 2, quantum: int = 64,
+                 gpu: int = 0) -> None:
+        self._lib = lib()
+        h = C.c_void_p()
+        code = self._lib.pscbfm_server_create(int(gpu), int(lattice_kind), int(hbm_budget), int(tile_capacity),
+                                              int(replica_capacity), int(block), int(slots), int(quantum),
+                                              C.byref(h))
+        _raise(code, (self._lib.pscbfm_create_error() or b"").decode())
+        self._h = h
+
+    def close(self) -> None:
+        h = getattr(self, "_h", None)
+        if h:
+            self._lib.pscbfm_server_destroy(h)
+            self._h = None
+
+    __del__ = close
+
+    def _call(self, name: str, *args) -> None:
+        code = getattr(self._lib, name)(self._h, *args)
+        if code:
+            _raise(code, (self._lib.pscbfm_server_last_error(self._h) or b"").decode())
+
+    def submit(self, staged: "UpdaterGPUScBFM_AB_Type", n_mcs: int) -> int:
+        job = C.c_uint64()
+        self._call("pscbfm_server_submit", staged._h, int(n_mcs), C.byref(job))
+        return job.value
+
+    def wait(self, job: int, timeout: float = -1.0) -> bool:
+        done = C.c_int()
+        self._call("pscbfm_server_wait", int(job), float(timeout), C.byref(done))
+        return bool(done.value)
+
+    def job(self, job: int) -> JobInfo:
+        out = JobInfo()
+        self._call("pscbfm_server_job", int(job), C.byref(out))
+        return out
+
+    def result(self, job: int, n: int) -> np.ndarray:
+        out = np.empty((n, 3), dtype=np.int32)
+        self._call("pscbfm_server_result", int(job), _ptr(out, C.c_int32), int(n))
+        return out
+
+    def release(self, job: int) -> None:
+        self._call("pscbfm_server_release", int(job))
+
+    def drain(self) -> None:
+        self._call("pscbfm_server_drain")
+
+    def stats(self) -> ServerCounters:
+        out = ServerCounters()
+        self._call("pscbfm_server_stats", C.byref(out))
+        return out
+
+    def now(self) -> float:
+        t = C.c_double()
+        self._call("pscbfm_server_now", C.byref(t))
+        return t.value
 
 
 # -- host helpers -----------------------------------------------------------------

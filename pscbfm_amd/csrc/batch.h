@@ -63,6 +63,7 @@ public:
     uint32_t liveReplicas() const;
     uint32_t usedTiles() const { return usedTiles_; }
     uint32_t tileCapacity() const { return tileCap_; }
+    uint32_t replicaCapacity() const { return replicaCap_; }
     uint64_t deviceBytes() const;
     const LaunchShape& shape() const { return shape_; }
     uint64_t launchedSubsteps() const { return launched_; }

@@ -4,6 +4,7 @@
 
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <new>
 #include <stdexcept>
 #include <string>
@@ -14,6 +15,7 @@
 #include "coloring.h"
 #include "common.h"
 #include "generators.h"
+#include "server.h"
 #include "updater.h"
 
 struct pscbfm_updater {
@@ -23,6 +25,12 @@ struct pscbfm_updater {
 
 struct pscbfm_batch {
     std::unique_ptr<pscbfm::Batch> engine;
+    std::string error;
+};
+
+struct pscbfm_server {
+    std::unique_ptr<pscbfm::Server> engine;
+    std::mutex errorMutex;
     std::string error;
 };
 
@@ -68,6 +76,16 @@ template <class F>
 int batch_guarded(pscbfm_batch* b, F&& f) {
     if (!b || !b->engine) return PSCBFM_NULL_HANDLE;
     return translated(b->error, [&] { f(*b->engine); });
+}
+
+template <class F>
+int server_guarded(pscbfm_server* s, F&& f) {
+    if (!s || !s->engine) return PSCBFM_NULL_HANDLE;
+    std::string err;
+    const int rc = translated(err, [&] { f(*s->engine); });
+    std::lock_guard<std::mutex> lock(s->errorMutex);
+    s->error = err;
+    return rc;
 }
 
 template <class F>
@@ -571,6 +589,101 @@ int pscbfm_batch_usage(pscbfm_batch* b, uint32_t* live, uint32_t* used_tiles, ui
         if (tile_capacity) *tile_capacity = e.tileCapacity();
         if (device_bytes) *device_bytes = e.deviceBytes();
     });
+}
+
+// ---- request loop ------------------------------------------------------------------
+
+int pscbfm_server_create(int gpu, int lattice_kind, uint64_t hbm_budget, uint32_t tile_capacity,
+                         uint32_t replica_capacity, uint32_t block, uint32_t slots, uint32_t quantum,
+                         pscbfm_server** out) {
+    if (!out) return PSCBFM_INVALID_ARGUMENT;
+    *out = nullptr;
+    if (gpu < 0) return PSCBFM_INVALID_ARGUMENT;
+    std::unique_ptr<pscbfm_server> s(new (std::nothrow) pscbfm_server());
+    if (!s) return PSCBFM_RUNTIME_ERROR;
+    const int rc = translated(g_createError, [&] {
+        s->engine.reset(new pscbfm::Server(gpu, uint32_t(lattice_kind), hbm_budget, tile_capacity, replica_capacity,
+                                           block, slots, quantum));
+    });
+    if (rc == PSCBFM_OK) *out = s.release();
+    return rc;
+}
+
+void pscbfm_server_destroy(pscbfm_server* s) { delete s; }
+
+const char* pscbfm_server_last_error(const pscbfm_server* s) { return s ? s->error.c_str() : "null handle"; }
+
+int pscbfm_server_submit(pscbfm_server* s, const pscbfm_updater* staged, uint64_t n_mcs, uint64_t* job) {
+    if (!staged) return PSCBFM_NULL_HANDLE;
+    return server_guarded(s, [&](pscbfm::Server& e) {
+        const uint64_t id = e.submit(staged->engine.staged(), n_mcs);
+        if (job) *job = id;
+    });
+}
+
+int pscbfm_server_wait(pscbfm_server* s, uint64_t job, double timeout_s, int* finished) {
+    return server_guarded(s, [&](pscbfm::Server& e) {
+        const bool done = e.wait(job, timeout_s);
+        if (finished) *finished = done ? 1 : 0;
+    });
+}
+
+int pscbfm_server_job(pscbfm_server* s, uint64_t job, pscbfm_job_info* info) {
+    if (!info) return PSCBFM_INVALID_ARGUMENT;
+    return server_guarded(s, [&](pscbfm::Server& e) {
+        const pscbfm::JobRecord r = e.record(job);
+        info->state = int32_t(r.state);
+        info->n_monomers = r.nMonomers;
+        info->n_mcs = r.nMCS;
+        info->accepted = r.accepted;
+        info->device_bytes = r.footprint;
+        info->t_submit = r.tSubmit;
+        info->t_admit = r.tAdmit;
+        info->t_done = r.tDone;
+        if (r.state == pscbfm::JobState::Failed) throw std::runtime_error("job failed: " + r.error);
+    });
+}
+
+int pscbfm_server_result(pscbfm_server* s, uint64_t job, int32_t* xyz, uint32_t n) {
+    if (!xyz && n) return PSCBFM_INVALID_ARGUMENT;
+    return server_guarded(s, [&](pscbfm::Server& e) {
+        const pscbfm::JobRecord r = e.record(job);
+        if (r.state != pscbfm::JobState::Done) throw std::runtime_error("job has no result (not done)");
+        if (size_t(n) * 3 != r.xyz.size()) throw std::invalid_argument("count differs from the job's monomers");
+        std::memcpy(xyz, r.xyz.data(), r.xyz.size() * sizeof(int32_t));
+    });
+}
+
+int pscbfm_server_release(pscbfm_server* s, uint64_t job) {
+    return server_guarded(s, [&](pscbfm::Server& e) { e.release(job); });
+}
+
+int pscbfm_server_drain(pscbfm_server* s) {
+    return server_guarded(s, [&](pscbfm::Server& e) { e.drain(); });
+}
+
+int pscbfm_server_stats(pscbfm_server* s, pscbfm_server_counters* out) {
+    if (!out) return PSCBFM_INVALID_ARGUMENT;
+    return server_guarded(s, [&](pscbfm::Server& e) {
+        const pscbfm::ServerStats st = e.stats();
+        out->submitted = st.submitted;
+        out->done = st.done;
+        out->failed = st.failed;
+        out->queued = st.queued;
+        out->running = st.running;
+        out->resident_bytes = st.residentBytes;
+        out->peak_resident_bytes = st.peakResidentBytes;
+        out->used_tiles = st.usedTiles;
+        out->tile_capacity = st.tileCapacity;
+        out->launches = st.launches;
+        out->substeps = st.substeps;
+        out->gpu_busy_seconds = st.gpuBusySeconds;
+    });
+}
+
+int pscbfm_server_now(pscbfm_server* s, double* seconds) {
+    if (!seconds) return PSCBFM_INVALID_ARGUMENT;
+    return server_guarded(s, [&](pscbfm::Server& e) { *seconds = e.now(); });
 }
 
 const char* pscbfm_version(void) { return "pscbfm-mi355x 0.1 (gfx950)"; }

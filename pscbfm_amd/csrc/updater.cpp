@@ -293,20 +293,39 @@ SystemInput Updater::stagedInput() const {
     return in;
 }
 
-// Checks of UpdaterGPUScBFM_AB_Type.cu:514-530 (108 allowed bonds) and the
-// optional invariant check, then the species layout.
-SystemLayout Updater::buildLayout() const {
-    const SystemInput in = stagedInput();
+namespace {
+void check_bond_set(const uint8_t forbidden[512]) {
     unsigned nAllowed = 0;
-    for (int i = 0; i < 512; ++i) nAllowed += forbidden_[i] ? 0u : 1u;
+    for (int i = 0; i < 512; ++i) nAllowed += forbidden[i] ? 0u : 1u;
     if (nAllowed != 108) {
         std::ostringstream msg;
         msg << "[UpdaterGPUScBFM_AB_Type::initialize] Wrong bond-set! Expected 108 allowed bonds, but got "
             << nAllowed;
         throw std::runtime_error(msg.str());
     }
+}
+}  // namespace
+
+SystemLayout layout_of(const StagedSystem& s) {
+    check_bond_set(s.input.forbidden);
+    return build_layout(s.input, s.colors.empty() ? nullptr : &s.colors, s.uniformColors);
+}
+
+StagedSystem Updater::staged() const {
+    StagedSystem s;
+    s.input = stagedInput();
+    if (hasColorOverride_) s.colors = colorOverride_;
+    s.uniformColors = uniformColors_;
+    s.seed = seed_;
+    return s;
+}
+
+// The 108-bond check, the optional invariant check, then the species layout.
+SystemLayout Updater::buildLayout() const {
+    StagedSystem s = staged();
+    check_bond_set(s.input.forbidden);
     if (checkOnInit_) checkSystem();
-    return build_layout(in, hasColorOverride_ ? &colorOverride_ : nullptr, uniformColors_);
+    return layout_of(s);
 }
 
 void Updater::initialize() {

@@ -29,6 +29,18 @@ struct RunStats {
     uint64_t performLaunches = 0;
 };
 
+// A system as staged through the setters, detached from its updater: what a
+// batch or a request queue needs to build the layout later, on any thread.
+struct StagedSystem {
+    SystemInput input;
+    std::vector<uint8_t> colors;  // empty: automatic coloring
+    bool uniformColors = true;
+    uint64_t seed = 0;
+};
+
+// The 108-bond check of UpdaterGPUScBFM_AB_Type.cu:514-530, then the layout.
+SystemLayout layout_of(const StagedSystem& s);
+
 class Updater {
 public:
     Updater();
@@ -83,6 +95,7 @@ public:
     // initialize() would build from it (used to hand a system to a batch).
     SystemInput stagedInput() const;
     SystemLayout buildLayout() const;
+    StagedSystem staged() const;
     bool initialized() const { return initialized_; }
     uint32_t latticeKind() const { return latticeKind_; }
     const LaunchShape& launchShape() const { return shape_; }
