@@ -164,6 +164,20 @@ int pscbfm_color_graph(uint32_t n, const uint32_t* start, const uint32_t* ids, i
 /* The 108 scBFM bond vectors (3 ints each) in a fixed order. */
 int pscbfm_standard_bond_vectors(int32_t* xyz108);
 
+/* ---- BFM files (LeMonADE's format, the reference's I/O) --------------------- */
+
+/* Reads the last configuration of a BFM file (plain or gzip) and stages it
+ * with the call sequence of GPUScBFM_AB_Type.h:83-130: periodicity, monomer
+ * count, coordinates, attributes, both directions of every bond, box,
+ * populateLattice, and the file's !set_of_bondvectors as the allowed bonds
+ * (copyBondSet over [-4,3]^3).  *mcs (optional) receives the frame's age. */
+int pscbfm_bfm_stage(pscbfm_updater* h, const char* path, uint64_t* mcs);
+/* Writes the updater's current coordinates as a frame with age `mcs`.
+ * append = 0: a new file (header with topology, then the frame); append != 0:
+ * appends the frame, writing the header first if the file is empty or
+ * missing (as AnalyzerWriteBfmFile appends every save interval). */
+int pscbfm_bfm_write(const pscbfm_updater* h, const char* path, uint64_t mcs, int append);
+
 /* ---- Synthetic systems (host only, seeded, deterministic) ------------------ */
 
 /* Linear chains grown as self-avoiding walks with bonds drawn uniformly from

@@ -173,6 +173,8 @@ _SIGNATURES = {
     "pscbfm_server_drain": (C.c_int, [_H]),
     "pscbfm_server_stats": (C.c_int, [_H, C.POINTER(ServerCounters)]),
     "pscbfm_server_now": (C.c_int, [_H, C.POINTER(C.c_double)]),
+    "pscbfm_bfm_stage": (C.c_int, [_H, C.c_char_p, _u64p]),
+    "pscbfm_bfm_write": (C.c_int, [_H, C.c_char_p, C.c_uint64, C.c_int]),
     "pscbfm_version": (C.c_char_p, []),
 }
 
@@ -346,6 +348,17 @@ class UpdaterGPUScBFM_AB_Type:
         ms = (C.c_double * len(TUNE_CANDIDATES))()
         self._call("pscbfm_autotune", int(mcs), C.byref(b), C.byref(sl), ms)
         return b.value, sl.value, {c: ms[i] for i, c in enumerate(TUNE_CANDIDATES) if ms[i] >= 0}
+
+    def loadBfm(self, path) -> int:
+        """Stages the last configuration of a BFM file (reference call order);
+        returns the frame's MCS age."""
+        mcs = C.c_uint64()
+        self._call("pscbfm_bfm_stage", str(path).encode(), C.byref(mcs))
+        return mcs.value
+
+    def writeBfm(self, path, mcs: int, append: bool = False) -> None:
+        """Writes the current coordinates as a BFM frame (header first unless appending to a non-empty file)."""
+        self._call("pscbfm_bfm_write", str(path).encode(), int(mcs), int(bool(append)))
 
     def lastRunStats(self) -> RunStats:
         s = RunStats()

@@ -55,6 +55,22 @@ class BfmSystem:
         return start, b.astype(np.uint32)
 
 
+_SEEDS = [(2, 0, 0), (2, 1, 0), (2, -1, 0), (1, 2, 0), (1, -2, 0), (2, 1, 1), (2, -1, 1), (2, 1, -1), (2, -1, -1),
+          (3, 0, 0), (2, 2, 1), (2, -2, 1), (2, 2, -1), (2, -2, -1), (3, 1, 0), (3, -1, 0), (1, 3, 0), (1, -3, 0)]
+
+
+def lemonade_bond_ids() -> Dict[int, Tuple[int, int, int]]:
+    """The ids LeMonADE writes for the 108 scBFM vectors (17..124): each of 18
+    seed vectors (a,b,c) followed by (b,c,a), (c,a,b), (-a,b,c), (b,c,-a), (c,-a,b)."""
+    out = {}
+    k = 17
+    for a, b, c in _SEEDS:
+        for v in ((a, b, c), (b, c, a), (c, a, b), (-a, b, c), (b, c, -a), (c, -a, b)):
+            out[k] = v
+            k += 1
+    return out
+
+
 def _open(path: Path) -> bytes:
     data = Path(path).read_bytes()
     if data[:2] == b"\x1f\x8b":
@@ -152,11 +168,8 @@ def write_bfm(path, box, xyz: np.ndarray, bonds: np.ndarray, bondset: Optional[D
               attributes: Optional[np.ndarray] = None, mcs: int = 0, periodic=(True, True, True)) -> None:
     """Writes one frame.  Bonds (i, i+1) whose vector is in the bond set are
     written inside strands; every other bond goes to !bonds."""
-    from . import standard_bond_vectors  # the 108-set with ids 17.. in a fixed order
-
     if bondset is None:
-        vecs = standard_bond_vectors()
-        bondset = {17 + k: tuple(int(c) for c in vecs[k]) for k in range(len(vecs))}
+        bondset = lemonade_bond_ids()
     code_of = {v: k for k, v in bondset.items()}
     xyz = np.asarray(xyz, dtype=np.int64).reshape(-1, 3)
     n = xyz.shape[0]

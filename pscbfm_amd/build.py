@@ -60,13 +60,22 @@ def build(verbose: bool = False) -> Path:
     with cf.ThreadPoolExecutor(max_workers=min(8, len(sources))) as ex:
         objs = list(ex.map(_compile, sources))
     tmp = OUT.with_suffix(".so.tmp")
-    cmd = [_hipcc(), "-shared", f"--offload-arch={ARCH}", "-fPIC", *map(str, objs), "-o", str(tmp)]
+    cmd = [_hipcc(), "-shared", f"--offload-arch={ARCH}", "-fPIC", *map(str, objs), "-lz", "-o", str(tmp)]
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         raise RuntimeError(f"link failed: {' '.join(cmd)}\n{res.stdout}\n{res.stderr}")
     os.replace(tmp, OUT)
+    # command-line simulator over the C ABI (host code only)
+    cli_src = HERE / "cli" / "pscbfm_run.cpp"
+    cli = HERE / "pscbfm_run"
+    cmd = [shutil.which("g++") or "g++", "-O2", "-std=c++17", "-Wall", "-Wextra", "-I", str(INCLUDE), str(cli_src),
+           "-o", str(cli), "-L", str(HERE), "-lpscbfm", f"-Wl,-rpath,$ORIGIN", f"-Wl,-rpath-link,{ROCM}/lib",
+           "-Wl,--allow-shlib-undefined"]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"CLI build failed: {' '.join(cmd)}\n{res.stdout}\n{res.stderr}")
     if verbose:
-        print(f"built {OUT}")
+        print(f"built {OUT} and {cli}")
     return OUT
 
 

@@ -3,6 +3,7 @@
 #include "pscbfm.h"
 
 #include <cstring>
+#include <fstream>
 #include <memory>
 #include <mutex>
 #include <new>
@@ -12,6 +13,7 @@
 
 #include "autotune.h"
 #include "batch.h"
+#include "bfm_io.h"
 #include "coloring.h"
 #include "common.h"
 #include "generators.h"
@@ -324,6 +326,69 @@ int pscbfm_count_proposal_bits(const pscbfm_updater* h, uint64_t* bits_even, uin
         for (size_t i = half; i < w.size(); ++i) b += uint64_t(__builtin_popcount(w[i]));
         if (bits_even) *bits_even = a;
         if (bits_odd) *bits_odd = b;
+    });
+}
+
+int pscbfm_bfm_stage(pscbfm_updater* h, const char* path, uint64_t* mcs) {
+    return guarded(h, [&](pscbfm::Updater& u) {
+        if (!path) throw std::invalid_argument("null path");
+        const pscbfm::BfmData f = pscbfm::read_bfm(path);
+        const uint32_t n = f.nMonomers();
+        u.setPeriodicity(f.periodic[0], f.periodic[1], f.periodic[2]);
+        u.setNrOfAllMonomers(n);
+        u.setMonomerCoordinatesBulk(f.xyz.data(), n);
+        for (uint32_t i = 0; i < n; ++i) u.setAttribute(i, f.attributes[i]);
+        // both directions, grouped by source monomer as LeMonADE's molecule lists are
+        std::vector<std::vector<uint32_t>> nb(n);
+        for (const auto& b : f.bonds) {
+            nb[b.first].push_back(b.second);
+            nb[b.second].push_back(b.first);
+        }
+        for (uint32_t i = 0; i < n; ++i)
+            for (uint32_t j : nb[i]) u.setConnectivity(i, j);
+        u.setLatticeSize(f.box[0], f.box[1], f.box[2]);
+        u.populateLattice();
+        std::vector<uint8_t> allowed(512, 0);
+        for (const auto& kv : f.bondset) {
+            const auto& v = kv.second;
+            allowed[uint32_t(v[0] & 7) | uint32_t(v[1] & 7) << 3 | uint32_t(v[2] & 7) << 6] = 1;
+        }
+        for (int dx = -4; dx < 4; ++dx)
+            for (int dy = -4; dy < 4; ++dy)
+                for (int dz = -4; dz < 4; ++dz)
+                    u.copyBondSet(dx, dy, dz, !allowed[uint32_t(dx & 7) | uint32_t(dy & 7) << 3 | uint32_t(dz & 7) << 6]);
+        if (mcs) *mcs = f.mcs;
+    });
+}
+
+int pscbfm_bfm_write(const pscbfm_updater* h, const char* path, uint64_t mcs, int append) {
+    return guarded_const(h, [&](const pscbfm::Updater& u) {
+        if (!path) throw std::invalid_argument("null path");
+        const pscbfm::SystemInput in = u.stagedInput();
+        const uint32_t n = uint32_t(in.xyz.size() / 3);
+        std::vector<std::pair<uint32_t, uint32_t>> bonds;
+        for (uint32_t i = 0; i < n; ++i)
+            for (uint32_t k = in.bonds.start[i]; k < in.bonds.start[i + 1]; ++k)
+                if (i < in.bonds.ids[k]) bonds.emplace_back(i, in.bonds.ids[k]);
+        std::map<int, std::array<int32_t, 3>> ids;
+        for (const auto& kv : pscbfm::standard_bondset_ids()) {
+            const auto& v = kv.second;
+            if (!u.forbidden(uint32_t(v[0] & 7) | uint32_t(v[1] & 7) << 3 | uint32_t(v[2] & 7) << 6)) ids[kv.first] = v;
+        }
+        const bool periodic[3] = {true, true, true};
+        pscbfm::BfmWriter w(u.box(), periodic, n, bonds, ids, u.attributes());
+        std::vector<int32_t> xyz(size_t(n) * 3);
+        u.getPositionsBulk(xyz.data(), n);
+        bool needHeader = !append;
+        if (append) {
+            std::ifstream probe(path, std::ios::binary | std::ios::ate);
+            needHeader = !probe || probe.tellg() <= 0;
+        }
+        std::ofstream out(path, needHeader ? std::ios::binary | std::ios::trunc : std::ios::binary | std::ios::app);
+        if (!out) throw std::runtime_error(std::string("cannot write ") + path);
+        if (needHeader) w.header(out);
+        w.frame(out, mcs, xyz.data());
+        if (!out) throw std::runtime_error(std::string("write failed: ") + path);
     });
 }
 

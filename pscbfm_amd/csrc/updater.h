@@ -97,6 +97,8 @@ public:
     SystemLayout buildLayout() const;
     StagedSystem staged() const;
     bool initialized() const { return initialized_; }
+    const std::vector<int32_t>& attributes() const { return attr_; }
+    bool forbidden(uint32_t bondIndex) const { return forbidden_[bondIndex & 511u] != 0; }
     uint32_t latticeKind() const { return latticeKind_; }
     const LaunchShape& launchShape() const { return shape_; }
     const uint32_t* box() const { return box_; }

@@ -324,3 +324,50 @@ def test_star_network_degrees():
     deg = np.diff(start.astype(np.int64))
     assert deg.max() <= 4 and deg[0] == 4
     assert s.bonds.shape[0] > 20 * 4 * 29   # some crosslinks formed
+
+
+def test_native_bfm_reader_and_writer_match_python(tmp_path):
+    src = GOLDEN / "LinearChain_l1024_n64.bfm.gz"
+    ref = read_bfm(src)
+    u = P.UpdaterGPUScBFM_AB_Type()
+    assert u.loadBfm(src) == ref.mcs
+    assert np.array_equal(u.getPositions(ref.n), ref.xyz)
+    out = tmp_path / "native.bfm"
+    u.writeBfm(out, 7)
+    u.writeBfm(out, 9, append=True)
+    back = read_bfm(out)
+    assert back.mcs == 9 and [f[0] for f in back.frames] == [7, 9]
+    assert np.array_equal(back.xyz, ref.xyz) and back.box == ref.box
+    key = lambda arr: sorted(tuple(sorted(map(int, p))) for p in arr)
+    assert key(back.bonds) == key(ref.bonds)
+    assert np.array_equal(back.attributes, ref.attributes)
+    assert {k: tuple(v) for k, v in back.bondset.items()} == {k: tuple(v) for k, v in ref.bondset.items()}
+
+
+def test_native_bfm_reader_rejects_non_periodic_fixtures():
+    # the reference's setPeriodicity accepts only periodic boxes (UpdaterGPUScBFM_AB_Type.cu:728-753)
+    for name in ["Ring_N6_n6.bfm", "Hydrogel_NStar_117_NoPerXYZ64.bfm"]:
+        with pytest.raises(P.InvalidArgument):
+            P.UpdaterGPUScBFM_AB_Type().loadBfm(GOLDEN / name)
+
+
+def test_native_bfm_reader_errors(tmp_path):
+    bad = tmp_path / "bad.bfm"
+    bad.write_bytes(b"!number_of_monomers=2\n!box_x=16\n!box_y=16\n!box_z=16\n")
+    with pytest.raises(RuntimeError):
+        P.UpdaterGPUScBFM_AB_Type().loadBfm(bad)
+    with pytest.raises(RuntimeError):
+        P.UpdaterGPUScBFM_AB_Type().loadBfm(tmp_path / "missing.bfm")
+
+
+def test_cli_prints_help():
+    import subprocess
+    exe = Path(P.__file__).parent / "pscbfm_run"
+    res = subprocess.run([str(exe), "--help"], capture_output=True, text=True, timeout=60)
+    assert res.returncode == 0 and "--max-mcs" in res.stdout and "--save-interval" in res.stdout
+
+
+def test_lemonade_bond_ids_match_the_reference_fixture():
+    from pscbfm_amd.bfm import lemonade_bond_ids
+    ref = read_bfm(GOLDEN / "LinearChain_l1024_n64.bfm.gz").bondset
+    assert {k: tuple(v) for k, v in ref.items()} == lemonade_bond_ids()
