@@ -41,6 +41,37 @@ __device__ __forceinline__ uint32_t xcd_block() {
     return (blockIdx.x & 7u) * (nb >> 3) + (blockIdx.x >> 3);
 }
 
+// Every device buffer is reached through a global-address-space pointer.
+// The batch kernels read their buffer pointers from a descriptor table in
+// memory, which leaves the compiler generic (FLAT) pointers: on gfx950 the
+// waits it then places before using FLAT-loaded gather results proved
+// insufficient under load (a few lanes consumed words that had not arrived;
+// a full s_waitcnt after the gathers removed the divergence).  GLOBAL
+// instructions, which the single-system kernels always get, are tracked by
+// the vector-memory counter alone.
+template <class T>
+using gptr = __attribute__((address_space(1))) T*;
+template <class T>
+__device__ __forceinline__ gptr<T> global_ptr(T* p) {
+    return (gptr<T>)p;
+}
+
+// Species record, read word by word through a global pointer.
+__device__ __forceinline__ ColorInfo load_color(const ColorInfo* colors, uint32_t species) {
+    const gptr<const uint32_t> w = (gptr<const uint32_t>)global_ptr(colors + species);
+    ColorInfo c{};
+    c.slotOffset = w[offsetof(ColorInfo, slotOffset) / 4];
+    c.count = w[offsetof(ColorInfo, count) / 4];
+    c.nbrOffset = w[offsetof(ColorInfo, nbrOffset) / 4];
+    c.nbrPitch = w[offsetof(ColorInfo, nbrPitch) / 4];
+    c.maxDegree = w[offsetof(ColorInfo, maxDegree) / 4];
+    return c;
+}
+
+// A list entry as one 64-bit word: low = species-local index | direction
+// << kDirShift, high = old code.
+__device__ __forceinline__ uint64_t pack_entry(uint32_t x, uint32_t y) { return uint64_t(x) | (uint64_t(y) << 32); }
+
 struct Move {
     uint32_t to;        // code of the new corner
     uint32_t cell[9];   // the 3x3 plane in front of the move
@@ -81,7 +112,7 @@ __device__ __forceinline__ Move make_move(uint32_t m, uint32_t dir, const uint32
 // four 8-byte gathers replace nine narrow ones, and the middle row/column
 // takes the word of whichever outer row/column shares its block.
 template <int SHIFT>
-__device__ __forceinline__ uint32_t plane_any(const uint64_t* words, const Move& mv) {
+__device__ __forceinline__ uint32_t plane_any(gptr<const uint64_t> words, const Move& mv) {
     const uint64_t w00 = words[mv.cell[0] >> SHIFT];  // (u-1, v-1)
     const uint64_t w02 = words[mv.cell[2] >> SHIFT];  // (u-1, v+1)
     const uint64_t w20 = words[mv.cell[6] >> SHIFT];  // (u+1, v-1)
@@ -112,27 +143,35 @@ constexpr uint32_t kDirShift = 29;
 // Pass 1 of substep t for logical workgroup lb of nb of one system.  The
 // survivors go to `list` (capacity SLOTS * blockDim), their number to *count.
 template <int LAT, int SLOTS, int MAXDEG>
-__device__ __forceinline__ void check_body(const SysDesc& d, uint64_t t, uint32_t lb, uint32_t nb, uint2* list,
-                                           uint32_t* count, uint32_t* sForbidden, uint32_t* sCount) {
+__device__ __forceinline__ void check_body(const SysDesc& d, uint64_t t, uint32_t lb, uint32_t nb, gptr<uint64_t> list,
+                                           gptr<uint32_t> count, uint32_t* sForbidden, uint32_t* sCount) {
     const SubstepDraw draw = substep_draw(d.seed, t, d.nColors);
-    const ColorInfo ci = d.colors[draw.species];
+    const ColorInfo ci = load_color(d.colors, draw.species);
+    const gptr<uint32_t> proposal = global_ptr(d.proposal);
 
-    // Zero the proposal buffer of the other parity, one slice per workgroup.
+    // Zero the proposal buffer of the other parity, one slice per workgroup
+    // (16 bytes per thread and step).
     {
-        uint4* prev = reinterpret_cast<uint4*>(d.proposal + size_t((t + 1u) & 1u) * d.latWords);
+        const gptr<uint64_t> prev = (gptr<uint64_t>)(proposal + size_t((t + 1u) & 1u) * d.latWords);
         const uint32_t quads = d.latWords >> 2;
         const uint32_t per = (quads + nb - 1) / nb;
         const uint32_t q0 = lb * per;
         const uint32_t q1 = min(q0 + per, quads);
-        for (uint32_t q = q0 + threadIdx.x; q < q1; q += blockDim.x) prev[q] = make_uint4(0u, 0u, 0u, 0u);
+        for (uint32_t q = q0 + threadIdx.x; q < q1; q += blockDim.x) {
+            prev[2 * size_t(q)] = 0u;
+            prev[2 * size_t(q) + 1] = 0u;
+        }
     }
-    if (threadIdx.x < 16) sForbidden[threadIdx.x] = d.bondMask[threadIdx.x];
+    if (threadIdx.x < 16) sForbidden[threadIdx.x] = global_ptr(d.bondMask)[threadIdx.x];
     if (threadIdx.x == 0) *sCount = 0;
     __syncthreads();
 
     const uint32_t B = blockDim.x;
     const uint32_t base = lb * (SLOTS * B);
-    const uint32_t* pos = d.pos + ci.slotOffset;
+    const gptr<const uint32_t> posAll = global_ptr(static_cast<const uint32_t*>(d.pos));
+    const gptr<const uint32_t> pos = posAll + ci.slotOffset;
+    const gptr<const uint32_t> nbr = global_ptr(d.nbr);
+    const gptr<const uint64_t> lattice = global_ptr(static_cast<const uint64_t*>(d.lattice));
 
     // The load phase is branch-free: lanes without a monomer (or without a
     // partner in a row) load from an in-bounds stand-in address and discard
@@ -150,7 +189,7 @@ __device__ __forceinline__ void check_body(const SysDesc& d, uint64_t t, uint32_
 #pragma unroll
         for (int r = 0; r < MAXDEG; ++r) {
             const bool row = uint32_t(r) < ci.maxDegree;
-            const uint32_t id = d.nbr[row ? ci.nbrOffset + size_t(r) * ci.nbrPitch + jj : 0u];
+            const uint32_t id = nbr[row ? ci.nbrOffset + size_t(r) * ci.nbrPitch + jj : 0u];
             partner[s][r] = (live[s] && row) ? id : kNoNeighbor;
         }
     }
@@ -164,8 +203,8 @@ __device__ __forceinline__ void check_body(const SysDesc& d, uint64_t t, uint32_
         to[s] = mv.to;
 #pragma unroll
         for (int r = 0; r < MAXDEG; ++r)
-            partnerCode[s][r] = d.pos[partner[s][r] != kNoNeighbor ? partner[s][r] : 0u];
-        occ[s] = plane_any<LAT == kLatticeU8 ? 3 : 6>(static_cast<const uint64_t*>(d.lattice), mv);
+            partnerCode[s][r] = posAll[partner[s][r] != kNoNeighbor ? partner[s][r] : 0u];
+        occ[s] = plane_any<LAT == kLatticeU8 ? 3 : 6>(lattice, mv);
     }
     bool ok[SLOTS];
 #pragma unroll
@@ -180,7 +219,7 @@ __device__ __forceinline__ void check_body(const SysDesc& d, uint64_t t, uint32_
     }
 
     // Mark targets and append survivors to this workgroup's list.
-    uint32_t* T = d.proposal + size_t(t & 1u) * d.latWords;
+    const gptr<uint32_t> T = proposal + size_t(t & 1u) * d.latWords;
     const uint32_t lane = threadIdx.x & 63u;
 #pragma unroll
     for (int s = 0; s < SLOTS; ++s) {
@@ -189,8 +228,8 @@ __device__ __forceinline__ void check_body(const SysDesc& d, uint64_t t, uint32_
         if (lane == 0 && mask != 0) waveBase = atomicAdd(sCount, uint32_t(__popcll(mask)));
         waveBase = __shfl(waveBase, 0, 64);
         if (ok[s]) {
-            atomicOr(&T[to[s] >> 5], 1u << (to[s] & 31u));
-            list[waveBase + lane_rank(mask)] = make_uint2(j[s] | (dir[s] << kDirShift), m[s]);
+            __hip_atomic_fetch_or(&T[to[s] >> 5], 1u << (to[s] & 31u), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            list[waveBase + lane_rank(mask)] = pack_entry(j[s] | (dir[s] << kDirShift), m[s]);
         }
     }
     __syncthreads();
@@ -212,28 +251,29 @@ __device__ __forceinline__ uint32_t block_sum(uint32_t v, uint32_t* scratch) {
 // Pass 2 of substep t over one pass-1 list.  The accepted count is added to
 // *accSlot, an entry only this workgroup writes.
 template <int LAT>
-__device__ __forceinline__ void perform_body(const SysDesc& d, uint64_t t, const uint2* list, uint32_t count,
-                                             uint64_t* accSlot, uint32_t* sSum) {
-    DevState* st = d.state;
+__device__ __forceinline__ void perform_body(const SysDesc& d, uint64_t t, gptr<const uint64_t> list, uint32_t count,
+                                             gptr<uint64_t> accSlot, uint32_t* sSum) {
     const SubstepDraw draw = substep_draw(d.seed, t, d.nColors);
-    const ColorInfo ci = d.colors[draw.species];
+    const ColorInfo ci = load_color(d.colors, draw.species);
     const uint32_t B = blockDim.x;
-    const uint64_t* T64 = reinterpret_cast<const uint64_t*>(d.proposal + size_t(t & 1u) * d.latWords);
+    const gptr<const uint64_t> T64 =
+        (gptr<const uint64_t>)(global_ptr(static_cast<const uint32_t*>(d.proposal)) + size_t(t & 1u) * d.latWords);
+    const gptr<uint32_t> pos = global_ptr(d.pos);
+    const gptr<uint8_t> lat = global_ptr(static_cast<uint8_t*>(d.lattice));
 
     // The list is compacted, so about a fifth of its capacity is filled: walk
     // only the entries that exist, one per thread and round.
     uint32_t accepted = 0;
     for (uint32_t idx = threadIdx.x; idx < count; idx += B) {
-        const uint2 e = list[idx];
-        const uint32_t jj = e.x & ((1u << kDirShift) - 1u);
-        const uint32_t dir = e.x >> kDirShift;
-        const uint32_t m = e.y;
+        const uint64_t e = list[idx];
+        const uint32_t jj = uint32_t(e) & ((1u << kDirShift) - 1u);
+        const uint32_t dir = uint32_t(e) >> kDirShift;
+        const uint32_t m = uint32_t(e >> 32);
         const Move mv = make_move(m, dir, d.mask);
         if (plane_any<6>(T64, mv)) continue;
         const uint32_t slot = ci.slotOffset + jj;
-        d.pos[slot] = mv.to;
+        pos[slot] = mv.to;
         if (LAT == kLatticeU8) {
-            uint8_t* lat = static_cast<uint8_t*>(d.lattice);
             lat[m] = 0;
             lat[mv.to] = 1;
         } else {
@@ -244,14 +284,13 @@ __device__ __forceinline__ void perform_body(const SysDesc& d, uint64_t t, const
             // any other monomer's old or new corner, so every byte touched
             // here has exactly one writer and its new value is known:
             // 0 for the vacated block, the single new bit for the entered one.
-            uint8_t* lat = static_cast<uint8_t*>(d.lattice);
             const uint32_t bFrom = m >> 3, bTo = mv.to >> 3;
             if (bFrom != bTo) lat[bFrom] = 0;
             lat[bTo] = uint8_t(1u << (mv.to & 7u));
         }
         if (mv.wraps) {
             const uint32_t axis = dir >> 1;
-            int32_t* img = axis == 0 ? d.imgX : (axis == 1 ? d.imgY : d.imgZ);
+            const gptr<int32_t> img = global_ptr(axis == 0 ? d.imgX : (axis == 1 ? d.imgY : d.imgZ));
             img[slot] += (dir & 1u) ? 1 : -1;
         }
         ++accepted;
@@ -259,9 +298,11 @@ __device__ __forceinline__ void perform_body(const SysDesc& d, uint64_t t, const
     const uint32_t total = block_sum(accepted, sSum);
     if (threadIdx.x == 0 && total != 0u) {
         *accSlot += total;
+        const gptr<DevState> st = global_ptr(d.state);
         if (st->log != nullptr) {
             const uint64_t idx = t - st->logFirst;
-            if (idx < st->logCapacity) atomicAdd(&st->log[idx], total);
+            if (idx < st->logCapacity)
+                __hip_atomic_fetch_add(&global_ptr(st->log)[idx], total, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
     }
 }
@@ -271,21 +312,25 @@ template <int LAT, int SLOTS, int MAXDEG>
 __global__ __launch_bounds__(256) void k_check(SysDesc d, uint32_t k) {
     __shared__ uint32_t sForbidden[16];
     __shared__ uint32_t sCount;
-    const uint64_t t = d.state->substep + k;
-    if (t >= d.state->stopAt) return;
+    const gptr<DevState> st = global_ptr(d.state);
+    const uint64_t t = st->substep + k;
+    if (t >= st->stopAt) return;
     const uint32_t lb = xcd_block();
-    check_body<LAT, SLOTS, MAXDEG>(d, t, lb, gridDim.x, d.list + size_t(lb) * (SLOTS * blockDim.x),
-                                   &d.listCount[lb], sForbidden, &sCount);
+    check_body<LAT, SLOTS, MAXDEG>(d, t, lb, gridDim.x,
+                                   global_ptr(reinterpret_cast<uint64_t*>(d.list)) + size_t(lb) * (SLOTS * blockDim.x),
+                                   global_ptr(d.listCount) + lb, sForbidden, &sCount);
 }
 
 template <int LAT, int SLOTS>
 __global__ __launch_bounds__(256) void k_perform(SysDesc d, uint32_t k) {
     __shared__ uint32_t sSum[4];
-    const uint64_t t = d.state->substep + k;
-    if (t >= d.state->stopAt) return;
+    const gptr<DevState> st = global_ptr(d.state);
+    const uint64_t t = st->substep + k;
+    if (t >= st->stopAt) return;
     const uint32_t lb = xcd_block();  // same numbering as pass 1: list lb
-    perform_body<LAT>(d, t, d.list + size_t(lb) * (SLOTS * blockDim.x), d.listCount[lb],
-                      &d.state->acceptedByBlock[blockIdx.x], sSum);
+    perform_body<LAT>(d, t,
+                      global_ptr(reinterpret_cast<const uint64_t*>(d.list)) + size_t(lb) * (SLOTS * blockDim.x),
+                      global_ptr(d.listCount)[lb], global_ptr(st->acceptedByBlock) + blockIdx.x, sSum);
 }
 
 // ---- batch of replicas ---------------------------------------------------------------
@@ -299,11 +344,12 @@ __global__ __launch_bounds__(256) void k_check_batch(const SysDesc* __restrict__
     const Tile tile = tiles[blockIdx.x];
     if (tile.replica == kNoReplica) return;
     const SysDesc& d = descs[tile.replica];
-    const uint64_t t = d.state->substep + k;
-    if (t >= d.state->stopAt) return;
+    const gptr<DevState> st = global_ptr(d.state);
+    const uint64_t t = st->substep + k;
+    if (t >= st->stopAt) return;
     check_body<LAT, SLOTS, MAXDEG>(d, t, tile.chunk, tile.chunks,
-                                   lists + size_t(blockIdx.x) * (SLOTS * blockDim.x), &listCounts[blockIdx.x],
-                                   sForbidden, &sCount);
+                                   global_ptr(reinterpret_cast<uint64_t*>(lists)) + size_t(blockIdx.x) * (SLOTS * blockDim.x),
+                                   global_ptr(listCounts) + blockIdx.x, sForbidden, &sCount);
 }
 
 template <int LAT, int SLOTS>
@@ -314,10 +360,11 @@ __global__ __launch_bounds__(256) void k_perform_batch(const SysDesc* __restrict
     const Tile tile = tiles[blockIdx.x];
     if (tile.replica == kNoReplica) return;
     const SysDesc& d = descs[tile.replica];
-    const uint64_t t = d.state->substep + k;
-    if (t >= d.state->stopAt) return;
-    perform_body<LAT>(d, t, lists + size_t(blockIdx.x) * (SLOTS * blockDim.x), listCounts[blockIdx.x],
-                      &d.state->acceptedByBlock[tile.chunk], sSum);
+    const gptr<DevState> st = global_ptr(d.state);
+    const uint64_t t = st->substep + k;
+    if (t >= st->stopAt) return;
+    perform_body<LAT>(d, t, global_ptr(reinterpret_cast<const uint64_t*>(lists)) + size_t(blockIdx.x) * (SLOTS * blockDim.x),
+                      global_ptr(listCounts)[blockIdx.x], global_ptr(st->acceptedByBlock) + tile.chunk, sSum);
 }
 
 __global__ void k_advance(DevState* st, uint32_t count) {
