@@ -70,16 +70,13 @@ void Batch::growTiles(uint32_t needed) {
     for (auto& f : freeSlots_) std::sort(f.begin(), f.end(), std::greater<uint32_t>());  // pop_back = lowest
     tileCap_ = needed;
     shape_.grid = tileCap_;
-    if (graph_) {
-        (void)hipGraphExecDestroy(graph_);
-        graph_ = nullptr;
-    }
+    dropGraphs();
     tablesDirty_ = true;
 }
 
 Batch::~Batch() {
     if (stream_) (void)hipStreamSynchronize(stream_);
-    if (graph_) (void)hipGraphExecDestroy(graph_);
+    dropGraphs();
     for (auto& r : replicas_) r.dev.reset();
     for (void* p : {static_cast<void*>(dDescs_), static_cast<void*>(dTiles_), static_cast<void*>(dLists_),
                     static_cast<void*>(dListCounts_), static_cast<void*>(dStops_), static_cast<void*>(dGather_)})
@@ -153,10 +150,7 @@ uint32_t Batch::add(const SystemLayout& L, uint64_t seed) {
     const uint32_t bucket = degree_bucket(r.dev->desc.maxDegree);
     if (bucket > graphDegree_) {
         graphDegree_ = bucket;
-        if (graph_) {
-            (void)hipGraphExecDestroy(graph_);
-            graph_ = nullptr;
-        }
+        dropGraphs();
     }
     tablesDirty_ = true;
     return id;
@@ -198,6 +192,15 @@ void Batch::uploadTables() {
         // the host tables may change as soon as this returns
         hip_check(hipStreamSynchronize(stream_), "sync tables");
         tablesDirty_ = false;
+        uint32_t high = 0;
+        for (uint32_t w = tileCap_; w-- > 0;)
+            if (tiles_[w].replica != kNoReplica) {
+                high = w + 1;
+                break;
+            }
+        uint32_t grid = 8;
+        while (grid < high) grid *= 2;
+        activeGrid_ = std::min(grid, tileCap_);
     }
     if (stopsDirty_) {
         std::vector<uint64_t> stops(replicaCap_, 0);
@@ -211,41 +214,51 @@ void Batch::uploadTables() {
     }
 }
 
-void Batch::captureGraph() {
-    if (graph_) {
-        (void)hipGraphExecDestroy(graph_);
-        graph_ = nullptr;
-    }
-    graphSubsteps_ = 16;
-    BatchLaunch b{dDescs_, dTiles_, dLists_, dListCounts_, tileCap_, graphDegree_, latticeKind_};
+void Batch::dropGraphs() {
+    for (auto& kv : graphs_) (void)hipGraphExecDestroy(kv.second);
+    graphs_.clear();
+}
+
+// Graph of kGraphSubsteps substeps over workgroups [0, grid), then the
+// counter advance.
+hipGraphExec_t Batch::graphFor(uint32_t grid) {
+    auto it = graphs_.find(grid);
+    if (it != graphs_.end()) return it->second;
+    BatchLaunch b{dDescs_, dTiles_, dLists_, dListCounts_, grid, graphDegree_, latticeKind_};
     hipGraph_t g = nullptr;
     hip_check(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
     hipError_t err = hipSuccess;
-    for (uint32_t k = 0; k < graphSubsteps_ && err == hipSuccess; ++k) {
+    for (uint32_t k = 0; k < kGraphSubsteps && err == hipSuccess; ++k) {
         err = launch_check_batch(b, k, shape_, stream_);
         if (err == hipSuccess) err = launch_perform_batch(b, k, shape_, stream_);
     }
-    if (err == hipSuccess) err = launch_advance_batch(dDescs_, replicaCap_, graphSubsteps_, stream_);
+    if (err == hipSuccess) err = launch_advance_batch(dDescs_, replicaCap_, kGraphSubsteps, stream_);
     const hipError_t endErr = hipStreamEndCapture(stream_, &g);
     if (err != hipSuccess) hip_fail(err, "kernel launch during capture");
     hip_check(endErr, "hipStreamEndCapture");
-    const hipError_t ie = hipGraphInstantiate(&graph_, g, nullptr, nullptr, 0);
+    hipGraphExec_t exec = nullptr;
+    const hipError_t ie = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
     (void)hipGraphDestroy(g);
     hip_check(ie, "hipGraphInstantiate");
+    graphs_[grid] = exec;
+    return exec;
 }
 
 void Batch::launch(uint64_t substeps) {
     hip_check(hipSetDevice(gpu_), "hipSetDevice");
     uploadTables();
     if (liveReplicas() == 0 || substeps == 0) return;
-    if (!graph_) captureGraph();
+    const uint32_t grid = activeGrid_;
     uint64_t left = substeps;
-    while (left >= graphSubsteps_) {
-        hip_check(hipGraphLaunch(graph_, stream_), "hipGraphLaunch");
-        left -= graphSubsteps_;
+    if (left >= kGraphSubsteps) {
+        hipGraphExec_t exec = graphFor(grid);
+        while (left >= kGraphSubsteps) {
+            hip_check(hipGraphLaunch(exec, stream_), "hipGraphLaunch");
+            left -= kGraphSubsteps;
+        }
     }
     if (left) {
-        BatchLaunch b{dDescs_, dTiles_, dLists_, dListCounts_, tileCap_, graphDegree_, latticeKind_};
+        BatchLaunch b{dDescs_, dTiles_, dLists_, dListCounts_, grid, graphDegree_, latticeKind_};
         for (uint32_t k = 0; k < left; ++k) {
             hip_check(launch_check_batch(b, k, shape_, stream_), "launch pass 1");
             hip_check(launch_perform_batch(b, k, shape_, stream_), "launch pass 2");

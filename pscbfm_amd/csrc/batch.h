@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <map>
 #include <memory>
 #include <vector>
 
@@ -79,7 +80,8 @@ private:
         uint64_t pendingStop = 0;
     };
     void uploadTables();
-    void captureGraph();
+    hipGraphExec_t graphFor(uint32_t grid);
+    void dropGraphs();
     void growTiles(uint32_t needed);
 
     int gpu_;
@@ -104,8 +106,12 @@ private:
     bool tablesDirty_ = true;
     bool stopsDirty_ = false;
     uint32_t graphDegree_ = 0;
-    hipGraphExec_t graph_ = nullptr;
-    uint32_t graphSubsteps_ = 0;
+    // Launches cover workgroups [0, activeGrid_): the highest occupied slot
+    // rounded up to a power of two, so a sparsely filled table does not pay
+    // for its idle workgroups.  One captured graph per grid size in use.
+    uint32_t activeGrid_ = 8;
+    std::map<uint32_t, hipGraphExec_t> graphs_;
+    static constexpr uint32_t kGraphSubsteps = 16;
     uint64_t launched_ = 0;
 };
 
