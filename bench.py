@@ -187,6 +187,7 @@ def main() -> int:
     ap.add_argument("--seed", type=int, default=42)
     ap.add_argument("--block", type=int, default=0, help="threads per workgroup (0: default)")
     ap.add_argument("--slots", type=int, default=0, help="monomers per pass-1 thread (0: default)")
+    ap.add_argument("--perform-block", type=int, default=0, help="pass-2 threads per workgroup (0: default)")
     ap.add_argument("--profile-mcs", type=int, default=20)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -217,6 +218,8 @@ def main() -> int:
     xyz = P.gen_linear_chains_ordered(box, n_chains, args.chain)
     bonds = P.chain_bonds(n_chains, args.chain)
     u = make_updater(box, xyz, bonds, args.seed, kind, args.block, args.slots)
+    if args.perform_block:
+        u.setPerformBlock(args.perform_block)
     colors = u.colors(xyz.shape[0])
     log(f"initialize done in {time.perf_counter() - tic:.1f} s; species sizes {u.colorSizes().tolist()}; "
         f"device bytes {u.deviceBytes() / 2**20:.0f} MiB")

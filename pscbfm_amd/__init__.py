@@ -125,6 +125,7 @@ _SIGNATURES = {
     "pscbfm_set_check_on_initialize": (C.c_int, [_H, C.c_int]),
     "pscbfm_set_colors": (C.c_int, [_H, _u8p, C.c_uint32]),
     "pscbfm_set_launch_shape": (C.c_int, [_H, C.c_uint32, C.c_uint32]),
+    "pscbfm_set_perform_block": (C.c_int, [_H, C.c_uint32]),
     "pscbfm_set_graph_chunk": (C.c_int, [_H, C.c_uint32]),
     "pscbfm_run_ex": (C.c_int, [_H, C.c_int32, _u32p, C.c_int, C.c_int, C.POINTER(RunStats)]),
     "pscbfm_last_run_stats": (C.c_int, [_H, C.POINTER(RunStats)]),
@@ -151,6 +152,7 @@ _SIGNATURES = {
     "pscbfm_batch_footprint": (C.c_int, [_H, _H, _u64p, _u32p]),
     "pscbfm_batch_add": (C.c_int, [_H, _H, _u32p]),
     "pscbfm_batch_remove": (C.c_int, [_H, C.c_uint32]),
+    "pscbfm_batch_set_perform_block": (C.c_int, [_H, C.c_uint32]),
     "pscbfm_batch_set_target": (C.c_int, [_H, C.c_uint32, C.c_uint64]),
     "pscbfm_batch_launch": (C.c_int, [_H, C.c_uint64]),
     "pscbfm_batch_synchronize": (C.c_int, [_H]),
@@ -316,6 +318,10 @@ class UpdaterGPUScBFM_AB_Type:
     def setLaunchShape(self, block: int = 0, slots: int = 0) -> None:
         self._call("pscbfm_set_launch_shape", int(block), int(slots))
 
+    def setPerformBlock(self, block: int = 0) -> None:
+        """Pass-2 threads per workgroup (0: the pass-1 block)."""
+        self._call("pscbfm_set_perform_block", int(block))
+
     def setGraphChunk(self, substeps: int) -> None:
         self._call("pscbfm_set_graph_chunk", int(substeps))
 
@@ -457,6 +463,9 @@ class Batch:
 
     def remove(self, rid: int) -> None:
         self._call("pscbfm_batch_remove", int(rid))
+
+    def set_perform_block(self, block: int) -> None:
+        self._call("pscbfm_batch_set_perform_block", int(block))
 
     def set_target(self, rid: int, n_mcs: int) -> None:
         self._call("pscbfm_batch_set_target", int(rid), int(n_mcs))

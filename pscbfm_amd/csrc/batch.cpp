@@ -84,6 +84,13 @@ Batch::~Batch() {
     if (stream_) (void)hipStreamDestroy(stream_);
 }
 
+void Batch::setPerformBlock(uint32_t block) {
+    if (block != 0 && (block % 64 != 0 || block > 256)) throw std::invalid_argument("perform block must be 64..256, multiple of 64");
+    hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+    shape_.performBlock = block;
+    dropGraphs();
+}
+
 uint32_t Batch::tilesNeeded(const SystemLayout& L) const {
     return std::max<uint32_t>(1u, tiles_for(L, shape_.block * shape_.slots));
 }

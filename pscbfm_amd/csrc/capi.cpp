@@ -248,6 +248,10 @@ int pscbfm_set_colors(pscbfm_updater* h, const uint8_t* colors, uint32_t n) {
     });
 }
 
+int pscbfm_set_perform_block(pscbfm_updater* h, uint32_t block) {
+    return guarded(h, [&](pscbfm::Updater& u) { u.setPerformBlock(block); });
+}
+
 int pscbfm_set_launch_shape(pscbfm_updater* h, uint32_t block, uint32_t slots) {
     return guarded(h, [&](pscbfm::Updater& u) { u.setLaunchShape(block, slots); });
 }
@@ -583,6 +587,10 @@ int pscbfm_batch_add(pscbfm_batch* b, const pscbfm_updater* staged, uint32_t* id
         const uint32_t r = e.add(staged->engine.buildLayout(), staged->engine.seed());
         if (id) *id = r;
     });
+}
+
+int pscbfm_batch_set_perform_block(pscbfm_batch* b, uint32_t block) {
+    return batch_guarded(b, [&](pscbfm::Batch& e) { e.setPerformBlock(block); });
 }
 
 int pscbfm_batch_remove(pscbfm_batch* b, uint32_t id) {

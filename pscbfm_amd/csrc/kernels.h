@@ -11,9 +11,12 @@ namespace pscbfm {
 // indices [w*slots*block, (w+1)*slots*block) and writes list w, which
 // workgroup w of pass 2 consumes.
 struct LaunchShape {
-    uint32_t block = 256;  // threads per workgroup (64..256, multiple of 64)
-    uint32_t slots = 2;    // monomers per pass-1 thread (1, 2 or 4)
-    uint32_t grid = 8;     // workgroups (multiple of 8)
+    uint32_t block = 256;        // pass-1 threads per workgroup (64..256, multiple of 64)
+    uint32_t slots = 2;          // monomers per pass-1 thread (1, 2 or 4)
+    uint32_t grid = 8;           // workgroups (multiple of 8)
+    uint32_t performBlock = 0;   // pass-2 threads per workgroup (0: the pass-1 block)
+    uint32_t passTwoBlock() const { return performBlock ? performBlock : block; }
+    uint32_t listStride() const { return slots * block; }  // list entries per workgroup
 };
 
 // One substep = pass 1 (propose/check) then pass 2 (resolve/apply).  `k` is

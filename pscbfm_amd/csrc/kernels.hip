@@ -2,7 +2,7 @@
 //
 // Pass 1 (check):   CRITICAL.cu:298-374  kernelSimulationScBFMCheckSpecies
 // Pass 2 (perform): CRITICAL.cu:376-411  kernelSimulationScBFMPerformSpeciesAndApply
-// Proposal clear:   CRITICAL.cu:561-573  (here folded into the next pass 1)
+// Proposal clear:   CRITICAL.cu:561-573  (here folded into pass 2, other parity)
 //
 // Design (wave64, CDNA4):
 //  * positions are Z-curve codes, so the own lattice cell is the code itself
@@ -19,8 +19,9 @@
 //    because the result of a substep does not depend on processing order),
 //    and pass 2 walks those lists densely;
 //  * the proposal lattice is 1 bit per cell and double-buffered by substep
-//    parity: pass 1 of substep t sets bits in buffer t&1 and zeroes buffer
-//    (t+1)&1, which pass 2 of substep t-1 was the last to read;
+//    parity: pass 1 of substep t sets bits in buffer t&1, pass 2 of substep t
+//    reads it and zeroes buffer (t+1)&1 (last read by pass 2 of substep t-1,
+//    next written by pass 1 of substep t+1);
 //  * the occupancy lattice is updated with plain byte stores (no atomics):
 //    device-scope atomics are executed beyond the per-XCD L2 and cost a
 //    memory transaction each;
@@ -140,28 +141,15 @@ __device__ __forceinline__ uint32_t lane_rank(uint64_t mask) {
 // List entry: x = species-local index | direction << 29, y = old code.
 constexpr uint32_t kDirShift = 29;
 
-// Pass 1 of substep t for logical workgroup lb of nb of one system.  The
+// Pass 1 of substep t for logical workgroup lb of one system.  The
 // survivors go to `list` (capacity SLOTS * blockDim), their number to *count.
 template <int LAT, int SLOTS, int MAXDEG>
-__device__ __forceinline__ void check_body(const SysDesc& d, uint64_t t, uint32_t lb, uint32_t nb, gptr<uint64_t> list,
+__device__ __forceinline__ void check_body(const SysDesc& d, uint64_t t, uint32_t lb, gptr<uint64_t> list,
                                            gptr<uint32_t> count, uint32_t* sForbidden, uint32_t* sCount) {
     const SubstepDraw draw = substep_draw(d.seed, t, d.nColors);
     const ColorInfo ci = load_color(d.colors, draw.species);
     const gptr<uint32_t> proposal = global_ptr(d.proposal);
 
-    // Zero the proposal buffer of the other parity, one slice per workgroup
-    // (16 bytes per thread and step).
-    {
-        const gptr<uint64_t> prev = (gptr<uint64_t>)(proposal + size_t((t + 1u) & 1u) * d.latWords);
-        const uint32_t quads = d.latWords >> 2;
-        const uint32_t per = (quads + nb - 1) / nb;
-        const uint32_t q0 = lb * per;
-        const uint32_t q1 = min(q0 + per, quads);
-        for (uint32_t q = q0 + threadIdx.x; q < q1; q += blockDim.x) {
-            prev[2 * size_t(q)] = 0u;
-            prev[2 * size_t(q) + 1] = 0u;
-        }
-    }
     if (threadIdx.x < 16) sForbidden[threadIdx.x] = global_ptr(d.bondMask)[threadIdx.x];
     if (threadIdx.x == 0) *sCount = 0;
     __syncthreads();
@@ -251,11 +239,29 @@ __device__ __forceinline__ uint32_t block_sum(uint32_t v, uint32_t* scratch) {
 // Pass 2 of substep t over one pass-1 list.  The accepted count is added to
 // *accSlot, an entry only this workgroup writes.
 template <int LAT>
-__device__ __forceinline__ void perform_body(const SysDesc& d, uint64_t t, gptr<const uint64_t> list, uint32_t count,
-                                             gptr<uint64_t> accSlot, uint32_t* sSum) {
+__device__ __forceinline__ void perform_body(const SysDesc& d, uint64_t t, uint32_t lb, uint32_t nb,
+                                             gptr<const uint64_t> list, uint32_t count, gptr<uint64_t> accSlot,
+                                             uint32_t* sSum) {
     const SubstepDraw draw = substep_draw(d.seed, t, d.nColors);
     const ColorInfo ci = load_color(d.colors, draw.species);
     const uint32_t B = blockDim.x;
+
+    // Zero the proposal buffer of the other parity, one slice per workgroup
+    // (16 bytes per thread and step): pass 2 of substep t-1 was its last
+    // reader and pass 1 of substep t+1 its next writer.  Streaming these
+    // stores here overlaps them with the latency-bound list walk.
+    {
+        const gptr<uint64_t> prev =
+            (gptr<uint64_t>)(global_ptr(d.proposal) + size_t((t + 1u) & 1u) * d.latWords);
+        const uint32_t quads = d.latWords >> 2;
+        const uint32_t per = (quads + nb - 1) / nb;
+        const uint32_t q0 = lb * per;
+        const uint32_t q1 = min(q0 + per, quads);
+        for (uint32_t q = q0 + threadIdx.x; q < q1; q += B) {
+            prev[2 * size_t(q)] = 0u;
+            prev[2 * size_t(q) + 1] = 0u;
+        }
+    }
     const gptr<const uint64_t> T64 =
         (gptr<const uint64_t>)(global_ptr(static_cast<const uint32_t*>(d.proposal)) + size_t(t & 1u) * d.latWords);
     const gptr<uint32_t> pos = global_ptr(d.pos);
@@ -316,20 +322,19 @@ __global__ __launch_bounds__(256) void k_check(SysDesc d, uint32_t k) {
     const uint64_t t = st->substep + k;
     if (t >= st->stopAt) return;
     const uint32_t lb = xcd_block();
-    check_body<LAT, SLOTS, MAXDEG>(d, t, lb, gridDim.x,
+    check_body<LAT, SLOTS, MAXDEG>(d, t, lb,
                                    global_ptr(reinterpret_cast<uint64_t*>(d.list)) + size_t(lb) * (SLOTS * blockDim.x),
                                    global_ptr(d.listCount) + lb, sForbidden, &sCount);
 }
 
-template <int LAT, int SLOTS>
-__global__ __launch_bounds__(256) void k_perform(SysDesc d, uint32_t k) {
+template <int LAT>
+__global__ __launch_bounds__(256) void k_perform(SysDesc d, uint32_t k, uint32_t listStride) {
     __shared__ uint32_t sSum[4];
     const gptr<DevState> st = global_ptr(d.state);
     const uint64_t t = st->substep + k;
     if (t >= st->stopAt) return;
     const uint32_t lb = xcd_block();  // same numbering as pass 1: list lb
-    perform_body<LAT>(d, t,
-                      global_ptr(reinterpret_cast<const uint64_t*>(d.list)) + size_t(lb) * (SLOTS * blockDim.x),
+    perform_body<LAT>(d, t, lb, gridDim.x, global_ptr(reinterpret_cast<const uint64_t*>(d.list)) + size_t(lb) * listStride,
                       global_ptr(d.listCount)[lb], global_ptr(st->acceptedByBlock) + blockIdx.x, sSum);
 }
 
@@ -347,15 +352,15 @@ __global__ __launch_bounds__(256) void k_check_batch(const SysDesc* __restrict__
     const gptr<DevState> st = global_ptr(d.state);
     const uint64_t t = st->substep + k;
     if (t >= st->stopAt) return;
-    check_body<LAT, SLOTS, MAXDEG>(d, t, tile.chunk, tile.chunks,
+    check_body<LAT, SLOTS, MAXDEG>(d, t, tile.chunk,
                                    global_ptr(reinterpret_cast<uint64_t*>(lists)) + size_t(blockIdx.x) * (SLOTS * blockDim.x),
                                    global_ptr(listCounts) + blockIdx.x, sForbidden, &sCount);
 }
 
-template <int LAT, int SLOTS>
+template <int LAT>
 __global__ __launch_bounds__(256) void k_perform_batch(const SysDesc* __restrict__ descs,
                                                        const Tile* __restrict__ tiles, const uint2* lists,
-                                                       const uint32_t* listCounts, uint32_t k) {
+                                                       const uint32_t* listCounts, uint32_t k, uint32_t listStride) {
     __shared__ uint32_t sSum[4];
     const Tile tile = tiles[blockIdx.x];
     if (tile.replica == kNoReplica) return;
@@ -363,7 +368,8 @@ __global__ __launch_bounds__(256) void k_perform_batch(const SysDesc* __restrict
     const gptr<DevState> st = global_ptr(d.state);
     const uint64_t t = st->substep + k;
     if (t >= st->stopAt) return;
-    perform_body<LAT>(d, t, global_ptr(reinterpret_cast<const uint64_t*>(lists)) + size_t(blockIdx.x) * (SLOTS * blockDim.x),
+    perform_body<LAT>(d, t, tile.chunk, tile.chunks,
+                      global_ptr(reinterpret_cast<const uint64_t*>(lists)) + size_t(blockIdx.x) * listStride,
                       global_ptr(listCounts)[blockIdx.x], global_ptr(st->acceptedByBlock) + tile.chunk, sSum);
 }
 
@@ -454,23 +460,14 @@ void check_by_slots(const SysDesc& d, const BatchLaunch* b, uint32_t k, const La
     }
 }
 
-template <int LAT, int SLOTS>
-void perform_launch(const SysDesc& d, const BatchLaunch* b, uint32_t k, const LaunchShape& s, hipStream_t st) {
-    const dim3 blk(s.block);
-    if (b)
-        hipLaunchKernelGGL((k_perform_batch<LAT, SLOTS>), dim3(b->tiles), blk, 0, st, b->descs, b->tileTable,
-                           b->lists, b->listCounts, k);
-    else
-        hipLaunchKernelGGL((k_perform<LAT, SLOTS>), dim3(s.grid), blk, 0, st, d, k);
-}
-
 template <int LAT>
-void perform_by_slots(const SysDesc& d, const BatchLaunch* b, uint32_t k, const LaunchShape& s, hipStream_t st) {
-    switch (s.slots) {
-        case 1: perform_launch<LAT, 1>(d, b, k, s, st); break;
-        case 2: perform_launch<LAT, 2>(d, b, k, s, st); break;
-        default: perform_launch<LAT, 4>(d, b, k, s, st); break;
-    }
+void perform_launch(const SysDesc& d, const BatchLaunch* b, uint32_t k, const LaunchShape& s, hipStream_t st) {
+    const dim3 blk(s.passTwoBlock());
+    if (b)
+        hipLaunchKernelGGL((k_perform_batch<LAT>), dim3(b->tiles), blk, 0, st, b->descs, b->tileTable, b->lists,
+                           b->listCounts, k, s.listStride());
+    else
+        hipLaunchKernelGGL((k_perform<LAT>), dim3(s.grid), blk, 0, st, d, k, s.listStride());
 }
 
 }  // namespace
@@ -485,9 +482,9 @@ hipError_t launch_check(const SysDesc& d, uint32_t k, const LaunchShape& s, hipS
 
 hipError_t launch_perform(const SysDesc& d, uint32_t k, const LaunchShape& s, hipStream_t st) {
     if (d.latticeKind == kLatticeU8)
-        perform_by_slots<kLatticeU8>(d, nullptr, k, s, st);
+        perform_launch<kLatticeU8>(d, nullptr, k, s, st);
     else
-        perform_by_slots<kLatticeBit>(d, nullptr, k, s, st);
+        perform_launch<kLatticeBit>(d, nullptr, k, s, st);
     return hipGetLastError();
 }
 
@@ -503,9 +500,9 @@ hipError_t launch_check_batch(const BatchLaunch& b, uint32_t k, const LaunchShap
 hipError_t launch_perform_batch(const BatchLaunch& b, uint32_t k, const LaunchShape& s, hipStream_t st) {
     const SysDesc none{};
     if (b.latticeKind == kLatticeU8)
-        perform_by_slots<kLatticeU8>(none, &b, k, s, st);
+        perform_launch<kLatticeU8>(none, &b, k, s, st);
     else
-        perform_by_slots<kLatticeBit>(none, &b, k, s, st);
+        perform_launch<kLatticeBit>(none, &b, k, s, st);
     return hipGetLastError();
 }
 

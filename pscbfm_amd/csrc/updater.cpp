@@ -205,9 +205,17 @@ void Updater::setLaunchShape(uint32_t block, uint32_t slots) {
     if (initialized_) applyLaunchShape();
 }
 
+void Updater::setPerformBlock(uint32_t block) {
+    if (block != 0 && (block % 64 != 0 || block > 256))
+        throw std::invalid_argument("[setPerformBlock] block must be 64, 128, 192 or 256");
+    performOverride_ = block;
+    if (initialized_) applyLaunchShape();
+}
+
 void Updater::applyLaunchShape() {
     shape_.block = blockOverride_ ? blockOverride_ : 256;
     shape_.slots = slotsOverride_ ? slotsOverride_ : 2;
+    shape_.performBlock = performOverride_;
     shape_.grid = grid_for(layout_.maxColorCount, shape_.block * shape_.slots);
     performGridMax_ = std::max(performGridMax_, shape_.grid);
     if (graphExec_) {

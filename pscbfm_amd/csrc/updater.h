@@ -76,6 +76,8 @@ public:
     // Threads per workgroup (0 = 256) and monomers per pass-1 thread (0 = 2).
     void setLaunchShape(uint32_t block, uint32_t slots);
     void setGraphChunk(uint32_t substeps) { graphChunk_ = substeps; }
+    // Pass-2 threads per workgroup (0 = the pass-1 block).
+    void setPerformBlock(uint32_t block);
     void setMonomerCoordinatesBulk(const int32_t* xyz, uint32_t n);
     void setConnectivityBulk(const uint32_t* pairs, uint64_t nPairs);
     void getPositionsBulk(int32_t* xyz, uint32_t n) const;
@@ -138,7 +140,7 @@ private:
     bool uniformColors_ = true;
     bool checkOnInit_ = true;
     uint32_t graphChunk_ = 0;   // 0: choose automatically
-    uint32_t blockOverride_ = 0, slotsOverride_ = 0;
+    uint32_t blockOverride_ = 0, slotsOverride_ = 0, performOverride_ = 0;
 
     // device
     bool initialized_ = false;

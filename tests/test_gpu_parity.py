@@ -83,3 +83,15 @@ def test_single_monomer_always_moves():
     stats, log = u.run(100, log=True)
     assert stats.accepted == 100 and np.all(log == 1)
     u.cleanup()
+
+
+@pytest.mark.parametrize("pblock", [64, 128, 192])
+def test_perform_block_does_not_change_results(pblock):
+    s = chains((64, 64, 64), 40, 64, seed=11)
+    u = make_updater(s, 5)
+    u.setPerformBlock(pblock)
+    _, log = u.run(15, log=True)
+    x = u.getPositions(s.n)
+    u.cleanup()
+    gx, glog, ox, olog = gpu_vs_oracle(s, seed=5, n_mcs=15)
+    assert np.array_equal(x, ox) and np.array_equal(log, olog)
