@@ -102,6 +102,62 @@ def cpu_baseline(u, box, bonds, colors, seed, target_seconds: float):
     }, parity, mcs
 
 
+def network_section(seed: int, mcs: int, kind: int, c_l: float) -> dict:
+    """Star network: 12800 4-arm stars of arm length 29 (1,497,600 monomers)
+    with arm ends crosslinked to other stars, on a 512x256x256 periodic box
+    (occupied fraction 0.357; the same monomer count on 256^3 would be 0.72,
+    beyond what seeded self-avoiding growth reaches).  Times `mcs` sweeps and
+    checks the first 3 sweeps of a second copy against the oracle."""
+    import oracle
+
+    box = (512, 256, 256)
+    tic = time.perf_counter()
+    xyz, bonds = P.gen_star_network(box, 12800, 4, 29, seed)
+    sd = W.SystemData(box, xyz, bonds, seed, "saw")
+    u = W.stage(sd, kind, initialize=True)
+    n = xyz.shape[0]
+    log(f"network: {n} monomers, {u.nColors()} species {u.colorSizes().tolist()}, "
+        f"built in {time.perf_counter() - tic:.1f} s")
+    block, slots, trials = u.autotune(3)
+    log(f"network autotune: block {block} slots {slots}; "
+        + ", ".join(f"{b}x{s_}:{ms:.3f}" for (b, s_), ms in sorted(trials.items())))
+    u.run(20)
+    st, _ = u.run(mcs)
+    p1, p2 = algorithmic_bytes(st, c_l)
+    value = st.attempted / (st.gpu_ms * 1e-3)
+    gbs = (p1 + p2) / (st.gpu_ms * 1e-3) / 1e9
+    log(f"network {mcs} MCS: {st.gpu_ms:.1f} ms GPU, {value / 1e9:.2f} G attempts/s, "
+        f"{gbs:.0f} GB/s algorithmic ({gbs / HBM_PEAK_GBS:.2f} of peak)")
+    u.cleanup()
+    # bit-exact check of the first sweeps from the initial state
+    v = W.stage(sd, kind, initialize=True)
+    colors = v.colors(n)
+    _, glog = v.run(3, log=True)
+    gx = v.getPositions(n)
+    v.cleanup()
+    pairs = W.directed_pairs(n, bonds)
+    start = np.concatenate([[0], np.cumsum(np.bincount(pairs[:, 0].astype(np.int64), minlength=n))]).astype(np.uint32)
+    ids = pairs[:, 1].copy()
+    o = oracle.System(box, xyz, start, ids, colors, seed=seed)
+    _, olog = o.run(0, 3 * (int(colors.max()) + 1))
+    parity = bool(np.array_equal(gx, o.xyz) and np.array_equal(glog, olog))
+    log(f"network parity over 3 MCS: {parity}")
+    return {
+        "workload": f"star network: 12800 4-arm stars (arm 29) = {n} monomers, arm ends crosslinked, "
+                    f"512x256x256 periodic, occupied fraction {8 * n / (512 * 256 * 256):.3f}, "
+                    f"{int(colors.max()) + 1} species, seed {seed}",
+        "value": value,
+        "unit": "attempted monomer moves/s",
+        "mcs": mcs,
+        "gpu_ms": st.gpu_ms,
+        "sweep_algorithmic_GBps": gbs,
+        "sweep_roofline_frac": gbs / HBM_PEAK_GBS,
+        "acceptance": st.accepted / st.attempted,
+        "shape": {"block": block, "slots": slots},
+        "parity_vs_oracle": f"{'bit-exact' if parity else 'MISMATCH'} over 3 MCS",
+    }
+
+
 def batched_vs_plain(n_rep: int, mcs: int, seed: int, kind: int, tune_sample: int, tune_mcs: int) -> dict:
     """Replica mix: one batch (autotuned shape, graph-replayed launches over
     every replica) against the plain loop of one updater per system, each
@@ -196,6 +252,8 @@ def main() -> int:
     ap.add_argument("--tune-sample", type=int, default=64)
     ap.add_argument("--tune-mcs", type=int, default=5)
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--network-mcs", type=int, default=2000)
+    ap.add_argument("--no-network", action="store_true")
     ap.add_argument("--serve-rates", default="20,80", help="Poisson arrival rates (jobs/s) of the served stream")
     ap.add_argument("--serve-seconds", type=float, default=8.0, help="arrival window per rate")
     ap.add_argument("--no-serve", action="store_true")
@@ -311,6 +369,8 @@ def main() -> int:
         out["parity_vs_oracle"] = f"{'bit-exact' if parity else 'MISMATCH'} over {mcs} MCS"
         log(f"CPU {cpu['value'] / 1e6:.1f} M attempts/s on {cpu['cores']} threads; parity {parity}")
     u.cleanup()
+    if not args.no_network and args.network_mcs > 0:
+        out["network"] = network_section(args.seed, args.network_mcs, kind, c_l)
     if not args.no_batch and args.replicas > 0:
         out["batched"] = batched_vs_plain(args.replicas, args.replica_mcs, args.seed, kind, args.tune_sample,
                                           args.tune_mcs)

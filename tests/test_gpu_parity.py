@@ -95,3 +95,32 @@ def test_perform_block_does_not_change_results(pblock):
     u.cleanup()
     gx, glog, ox, olog = gpu_vs_oracle(s, seed=5, n_mcs=15)
     assert np.array_equal(x, ox) and np.array_equal(log, olog)
+
+
+def test_network_config_bit_exact():
+    # Star network on 256^3 at the densest the seeded growth reaches: 7000
+    # 4-arm stars of arm length 29 (819000 monomers, occupied fraction 0.39),
+    # arm ends crosslinked; 3 species from the coloring.
+    s = stars((256, 256, 256), 7000, 4, 29, seed=3)
+    print("network", s.n, flush=True)
+    gx, glog, ox, olog = gpu_vs_oracle(s, seed=17, n_mcs=6)
+    assert np.array_equal(glog, olog)
+    assert np.array_equal(gx, ox)
+
+
+def test_plain_config_full_length_bit_exact():
+    # The plain single-system case in full: 64 SAW chains of 1024 on 256^3,
+    # seed 42, 10^4 MCS, compared with the oracle at the end.
+    s = chains((256, 256, 256), 64, 1024, seed=42)
+    print("plain full length", s.n, flush=True)
+    u = make_updater(s, 42)
+    u.runSimulationOnGPU(10000)
+    gx = u.getPositions(s.n)
+    _, acc = u.counters()
+    colors = u.colors(s.n)
+    u.cleanup()
+    from helpers import oracle_for
+    o = oracle_for(s, colors, 42)
+    _, olog = o.run(0, 10000 * (int(colors.max()) + 1))
+    assert np.array_equal(gx, o.xyz)
+    assert acc == int(olog.sum())
