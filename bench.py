@@ -158,7 +158,8 @@ def network_section(seed: int, mcs: int, kind: int, c_l: float) -> dict:
     }
 
 
-def batched_vs_plain(n_rep: int, mcs: int, seed: int, kind: int, tune_sample: int, tune_mcs: int) -> dict:
+def batched_vs_plain(n_rep: int, mcs: int, seed: int, kind: int, tune_sample: int, tune_mcs: int,
+                     plain: bool = True) -> dict:
     """Replica mix: one batch (autotuned shape, graph-replayed launches over
     every replica) against the plain loop of one updater per system, each
     called with runSimulationOnGPU(mcs) in turn.  Every replica's final
@@ -190,6 +191,11 @@ def batched_vs_plain(n_rep: int, mcs: int, seed: int, kind: int, tune_sample: in
     b_xyz = [batch.positions(i, r.n) for i, r in zip(ids, reps)]
     batch_wall = time.perf_counter() - tic
 
+    if not plain:  # batch only (profiling runs)
+        attempted_est = sum(r.n for r in reps) * mcs
+        log(f"batched {mcs} MCS: {batch_ms:.1f} ms GPU, {batch_wall * 1e3:.1f} ms wall incl. read-back")
+        return {"value": attempted_est / batch_wall, "unit": "monomer sweeps/s (plain loop skipped)",
+                "batch_ms": batch_ms, "shape": {"block": block, "slots": slots, "workgroups": used_tiles}}
     for u in staged:
         u.initialize()
     tic = time.perf_counter()
@@ -251,6 +257,7 @@ def main() -> int:
     ap.add_argument("--replica-mcs", type=int, default=500)
     ap.add_argument("--tune-sample", type=int, default=64)
     ap.add_argument("--tune-mcs", type=int, default=5)
+    ap.add_argument("--no-plain", action="store_true", help="batched section without the plain per-system loop")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--network-mcs", type=int, default=2000)
     ap.add_argument("--no-network", action="store_true")
@@ -373,7 +380,7 @@ def main() -> int:
         out["network"] = network_section(args.seed, args.network_mcs, kind, c_l)
     if not args.no_batch and args.replicas > 0:
         out["batched"] = batched_vs_plain(args.replicas, args.replica_mcs, args.seed, kind, args.tune_sample,
-                                          args.tune_mcs)
+                                          args.tune_mcs, plain=not args.no_plain)
     if not args.no_serve and args.serve_rates:
         from pscbfm_amd import serving
         out["serving"] = {}

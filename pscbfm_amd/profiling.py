@@ -17,18 +17,25 @@ from __future__ import annotations
 import argparse
 import csv
 import json
+import re
 from collections import defaultdict
 from pathlib import Path
 from typing import Dict, Optional
 
-KERNELS = {"k_check": "pass 1 (k_check)", "k_perform": "pass 2 (k_perform)", "k_copy_u4": "copy reference",
-           "k_advance": "counter advance"}
+KERNELS = {"k_check_batch": "pass 1, batch (k_check_batch)", "k_perform_batch": "pass 2, batch (k_perform_batch)",
+           "k_advance_batch": "counter advance, batch", "k_gather_state": "state gather, batch",
+           "k_set_stops": "stop points, batch", "k_check": "pass 1 (k_check)", "k_perform": "pass 2 (k_perform)",
+           "k_copy_u4": "copy reference", "k_advance": "counter advance", "k_fill_u32": "fill"}
+
+
+_NAME = re.compile(r"\b(k_[a-z0-9_]+?)(?=[<(]|$)")
 
 
 def _short(name: str) -> str:
-    for k in KERNELS:
-        if k in name:
-            return k
+    """Kernel symbol without namespace, template arguments or parameters."""
+    m = _NAME.search(name)
+    if m and m.group(1) in KERNELS:
+        return m.group(1)
     return name.split("(")[0].split("::")[-1][-40:] or name[:40]
 
 
@@ -104,10 +111,18 @@ def main(argv: Optional[list] = None) -> int:
     if stats_csv:
         lines += ["rocprofv3 --stats (all dispatches of the run):", "",
                   "| kernel | calls | average (us) | min (us) | max (us) | share % |", "|---|---|---|---|---|---|"]
-        for r in csv.DictReader(open(stats_csv[0])):
-            lines.append(f"| {KERNELS.get(_short(r['Name']), _short(r['Name']))} | {r['Calls']} | "
-                         f"{float(r['AverageNs']) / 1e3:.2f} | {float(r['MinNs']) / 1e3:.2f} | "
-                         f"{float(r['MaxNs']) / 1e3:.2f} | {float(r['Percentage']):.1f} |")
+        agg: Dict[str, dict] = {}
+        for r in csv.DictReader(open(stats_csv[0])):  # template instances summed per kernel
+            a = agg.setdefault(_short(r["Name"]), {"calls": 0, "total": 0.0, "min": float("inf"), "max": 0.0,
+                                                   "share": 0.0})
+            a["calls"] += int(r["Calls"])
+            a["total"] += float(r["AverageNs"]) * int(r["Calls"])
+            a["min"] = min(a["min"], float(r["MinNs"]))
+            a["max"] = max(a["max"], float(r["MaxNs"]))
+            a["share"] += float(r["Percentage"])
+        for k, a in sorted(agg.items(), key=lambda kv: -kv[1]["share"]):
+            lines.append(f"| {KERNELS.get(k, k)} | {a['calls']} | {a['total'] / a['calls'] / 1e3:.2f} | "
+                         f"{a['min'] / 1e3:.2f} | {a['max'] / 1e3:.2f} | {a['share']:.1f} |")
         lines.append("")
     lines += ["Kernel trace, steady state (mean over the last dispatches of each sweep kernel):", "",
               "| kernel | calls | mean duration (us) | over last | total (us) |", "|---|---|---|---|---|"]
