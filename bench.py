@@ -261,6 +261,7 @@ def main() -> int:
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--network-mcs", type=int, default=2000)
     ap.add_argument("--no-network", action="store_true")
+    ap.add_argument("--compare-mcs", type=int, default=300, help="MCS of the other lattice kind on the same melt")
     ap.add_argument("--serve-rates", default="20,80", help="Poisson arrival rates (jobs/s) of the served stream")
     ap.add_argument("--serve-seconds", type=float, default=8.0, help="arrival window per rate")
     ap.add_argument("--no-serve", action="store_true")
@@ -369,6 +370,24 @@ def main() -> int:
         "copy_reference_GBps": copy_gbs,
         "wall_ms_incl_copyback": wall * 1e3,
     }
+    if args.compare_mcs > 0:
+        # the same melt on the other occupancy lattice (uint8 vs 1-bit)
+        other = "u8" if args.lattice == "bit" else "bit"
+        okind = P.LATTICE_U8 if other == "u8" else P.LATTICE_BIT
+        oc_l = 1.0 if other == "u8" else 0.125
+        v = make_updater(box, xyz, bonds, args.seed, okind, args.block, args.slots)
+        v.run(min(args.warmup, 200))
+        ost, _ = v.run(args.compare_mcs)
+        v.cleanup()
+        op1, op2 = algorithmic_bytes(ost, oc_l)
+        ogbs = (op1 + op2) / (ost.gpu_ms * 1e-3) / 1e9
+        out["other_lattice"] = {
+            "lattice": other, "mcs": args.compare_mcs, "value": ost.attempted / (ost.gpu_ms * 1e-3),
+            "unit": "attempted monomer moves/s", "sweep_algorithmic_GBps": ogbs,
+            "sweep_roofline_frac": ogbs / HBM_PEAK_GBS,
+        }
+        log(f"{other} lattice: {out['other_lattice']['value'] / 1e9:.2f} G attempts/s, "
+            f"sweep {ogbs:.0f} GB/s algorithmic ({ogbs / HBM_PEAK_GBS:.2f} of peak)")
     if not args.no_cpu:
         log("CPU baseline (oracle) on a bounded sample ...")
         cpu, parity, mcs = cpu_baseline(u, box, bonds, colors, args.seed, args.cpu_seconds)
@@ -391,6 +410,9 @@ def main() -> int:
             res = serving.run_stream(jobs, log=log, seed=args.seed)
             log(f"  {res['jobs_per_s']:.1f} jobs/s, latency p50 {res['latency_p50_s']:.2f} s "
                 f"p99 {res['latency_p99_s']:.2f} s, GPU busy {res['gpu_busy_fraction']:.2f}; {res['parity']}")
+            plain = out.get("batched", {}).get("plain_value")
+            if plain:
+                res["speedup_vs_plain_loop"] = res["monomer_sweeps_per_s"] / plain
             out["serving"][f"{rate:g}/s"] = res
     print(json.dumps(out), flush=True)
     return 0

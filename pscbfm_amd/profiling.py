@@ -73,6 +73,15 @@ def counter_means(pmc_dir: Path, counter: str, last: Optional[int] = None) -> Di
     return {k: (lambda t: sum(t) / len(t))(_tail(v, last if k != "k_copy_u4" else None)) for k, v in acc.items()}
 
 
+def all_counters(pmc_dir: Path, last: Optional[int] = None) -> Dict[str, Dict[str, float]]:
+    """Per kernel, the mean of every counter of one --pmc run."""
+    rows = sorted(csv.DictReader(open(_find(pmc_dir, "counter_collection.csv"))), key=lambda r: int(r["Dispatch_Id"]))
+    acc: Dict[str, Dict[str, list]] = defaultdict(lambda: defaultdict(list))
+    for r in rows:
+        acc[_short(r["Kernel_Name"])][r["Counter_Name"]].append(float(r["Counter_Value"]))
+    return {k: {c: (lambda t: sum(t) / len(t))(_tail(v, last)) for c, v in cs.items()} for k, cs in acc.items()}
+
+
 def traffic(fetch_dir: Path, write_dir: Path, copy_bytes: int, last: Optional[int] = None) -> dict:
     fetch = counter_means(fetch_dir, "FETCH_SIZE", last)
     write = counter_means(write_dir, "WRITE_SIZE", last)
@@ -95,6 +104,8 @@ def main(argv: Optional[list] = None) -> int:
     ap.add_argument("--fetch", type=Path)
     ap.add_argument("--write", type=Path)
     ap.add_argument("--copy-bytes", type=int, default=1 << 29)
+    ap.add_argument("--counters", type=Path, nargs="*", default=[],
+                    help="directories of further --pmc runs; their counters are tabulated per sweep kernel")
     ap.add_argument("--md", type=Path)
     ap.add_argument("--json", type=Path)
     ap.add_argument("--command", default="")
@@ -136,13 +147,41 @@ def main(argv: Optional[list] = None) -> int:
         for k, v in tr["kernels"].items():
             lines.append(f"| {KERNELS.get(k, k)} | {v['fetch_kib']:.0f} | {v['write_kib']:.0f} | "
                          f"{v['read_bytes'] / 1e6:.1f} | {v['write_bytes'] / 1e6:.1f} | {v['traffic_bytes'] / 1e6:.1f} |")
+    counters: Dict[str, Dict[str, float]] = defaultdict(dict)
+    for d in args.counters:
+        for k, cs in all_counters(d, args.last).items():
+            counters[k].update(cs)
+    sweep = [k for k in ("k_check", "k_perform", "k_check_batch", "k_perform_batch") if k in counters]
+    if sweep:
+        names = sorted({c for k in sweep for c in counters[k]})
+        lines += ["", "Counters per dispatch (mean over the same dispatches):", "",
+                  "| counter | " + " | ".join(KERNELS[k] for k in sweep) + " |",
+                  "|---|" + "---|" * len(sweep)]
+        for c in names:
+            lines.append(f"| {c} | " + " | ".join(f"{counters[k].get(c, float('nan')):.4g}" for k in sweep) + " |")
+        derived = []
+        for k in sweep:
+            cs = counters[k]
+            row = {}
+            if "TCC_HIT_sum" in cs and "TCC_MISS_sum" in cs and cs["TCC_HIT_sum"] + cs["TCC_MISS_sum"] > 0:
+                row["L2 hit rate"] = cs["TCC_HIT_sum"] / (cs["TCC_HIT_sum"] + cs["TCC_MISS_sum"])
+            if "SQ_WAIT_INST_ANY" in cs and cs.get("SQ_WAVE_CYCLES", 0) > 0:
+                row["wave cycles waiting on memory"] = cs["SQ_WAIT_INST_ANY"] / cs["SQ_WAVE_CYCLES"]
+            if "TCP_TCC_READ_REQ_LATENCY_sum" in cs and cs.get("TCP_TCC_READ_REQ_sum", 0) > 0:
+                row["mean L1->L2 read latency (cycles)"] = cs["TCP_TCC_READ_REQ_LATENCY_sum"] / cs["TCP_TCC_READ_REQ_sum"]
+            derived.append(row)
+        keys = sorted({c for r in derived for c in r})
+        if keys:
+            lines += ["", "| derived | " + " | ".join(KERNELS[k] for k in sweep) + " |", "|---|" + "---|" * len(sweep)]
+            for c in keys:
+                lines.append(f"| {c} | " + " | ".join(f"{r.get(c, float('nan')):.3f}" for r in derived) + " |")
     text = "\n".join(lines) + "\n"
     if args.md:
         args.md.write_text(text)
     else:
         print(text)
     if args.json:
-        args.json.write_text(json.dumps({"durations": dur, "traffic": tr}, indent=1) + "\n")
+        args.json.write_text(json.dumps({"durations": dur, "traffic": tr, "counters": counters}, indent=1) + "\n")
     return 0
 
 
