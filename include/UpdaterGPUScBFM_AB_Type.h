@@ -5,7 +5,10 @@
  * reference class (src/pscbfm/UpdaterGPUScBFM_AB_Type.h:124-362), header-only
  * on top of the C ABI in pscbfm.h (link with -lpscbfm).  A caller written
  * against the reference (e.g. GPUScBFM_AB_Type.h:78-196) compiles unchanged
- * except for the include path; LeMonADE's RNG is replaced by setSeed().
+ * except for the include path.  The per-substep species and seed draws
+ * (CRITICAL.cu:509-510) come either from the caller's R250 state
+ * (setR250State, the reference's stream) or from a counter draw keyed by
+ * setSeed().
  *
  * Exceptions: std::invalid_argument, std::runtime_error and std::out_of_range
  * where the reference throws them; device failures, which make the reference
@@ -77,6 +80,13 @@ public:
     /* ---- extensions -------------------------------------------------------- */
     void checkSystem() { check(pscbfm_check_system(h_)); }
     void setSeed(uint64_t seed) { check(pscbfm_set_seed(h_, seed)); }
+    // R250 mode: the 250 state words in draw order, or a LeMonADE seed file
+    // (state = words 7..256) followed by `discard` draws.
+    void setR250State(const uint32_t* words250) { check(pscbfm_set_r250_state(h_, words250)); }
+    void loadR250SeedFile(const std::string& path, uint32_t discard) {
+        check(pscbfm_r250_from_seed_file(h_, path.c_str(), discard));
+    }
+    void getR250State(uint32_t* words250) { check(pscbfm_r250_get_state(h_, words250)); }
     void setLatticeKind(int kind) { check(pscbfm_set_lattice_kind(h_, kind)); }
     void setMonomerCoordinates(const std::vector<int32_t>& xyz) {
         check(pscbfm_set_monomer_coordinates_bulk(h_, xyz.data(), uint32_t(xyz.size() / 3)));

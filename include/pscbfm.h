@@ -93,6 +93,24 @@ int pscbfm_cleanup(pscbfm_updater* h);
 /* Key of the counter-based substep draw; replaces the seeding of LeMonADE's
  * R250 stream that drives CRITICAL.cu:509-510.  Before initialize. */
 int pscbfm_set_seed(pscbfm_updater* h, uint64_t seed);
+/* R250 mode: the per-substep species and kernel seed come from the R250
+ * stream the reference draws them from (CRITICAL.cu:509-510: species =
+ * r250 % nSpecies, then seed = r250), instead of the counter draw.  Any time;
+ * takes effect from the next run.  Batches and request queues reject an
+ * updater in R250 mode.
+ * set_r250_state: the 250 state words in draw order (RandomNumberGenerators
+ *   state as LeMonADE keeps it).
+ * r250_from_seed_file: RandomNumberGenerators::seedAll(file) as
+ *   mainCUDASimulatorGPUScBFM_AB_Type.cpp:129 calls it (state = words 7..256
+ *   of the file), then `discard` draws; the reference simulator discards 2
+ *   (its known-answer prints, mainCUDASimulatorGPUScBFM_AB_Type.cpp:138-139).
+ * r250_next: draws n raw outputs (out may be null to skip them).
+ * r250_get_state: the current 250 words in draw order (set_r250_state of
+ *   them continues the same stream). */
+int pscbfm_set_r250_state(pscbfm_updater* h, const uint32_t* words);
+int pscbfm_r250_from_seed_file(pscbfm_updater* h, const char* path, uint32_t discard);
+int pscbfm_r250_next(pscbfm_updater* h, uint32_t* out, uint64_t n);
+int pscbfm_r250_get_state(const pscbfm_updater* h, uint32_t* words);
 /* Occupancy lattice storage (pscbfm_lattice_kind).  Before initialize.
  * Replaces the USE_BIT_PACKING_LATTICE switch (UpdaterGPUScBFM_AB_Type.cu:13). */
 int pscbfm_set_lattice_kind(pscbfm_updater* h, int kind);

@@ -25,7 +25,12 @@ class OracleSystem(C.Structure):
     _fields_ = [
         ("box", C.c_uint32 * 3), ("n", C.c_uint32), ("xyz", _i32p), ("nbr_start", _u32p), ("nbr", _u32p),
         ("color", _u8p), ("n_colors", C.c_uint32), ("forbidden", _u8p), ("seed", C.c_uint64),
+        ("draws", _u32p), ("draw_t0", C.c_uint64),
     ]
+
+
+class R250State(C.Structure):
+    _fields_ = [("buf", C.c_uint32 * 250), ("index", C.c_uint32)]
 
 
 _LIB: Optional[C.CDLL] = None
@@ -69,6 +74,10 @@ def lib() -> C.CDLL:
         h.oracle_sequential_run.argtypes = [C.POINTER(OracleSystem), C.c_uint64, C.c_uint64]
         h.oracle_max_threads.restype = C.c_int
         h.oracle_max_threads.argtypes = []
+        h.oracle_r250_load.restype = None
+        h.oracle_r250_load.argtypes = [C.POINTER(R250State), _u32p]
+        h.oracle_r250_rand32.restype = C.c_uint32
+        h.oracle_r250_rand32.argtypes = [C.POINTER(R250State)]
         _LIB = h
     return _LIB
 
@@ -114,6 +123,34 @@ def max_threads() -> int:
     return int(lib().oracle_max_threads())
 
 
+class R250:
+    """The oracle's R250 (oracle_r250_*): load 250 state words, then draw."""
+
+    def __init__(self, state) -> None:
+        self._s = R250State()
+        arr = np.ascontiguousarray(state, dtype=np.uint32)
+        if arr.size != 250:
+            raise ValueError("R250 needs 250 state words")
+        lib().oracle_r250_load(C.byref(self._s), arr.ctypes.data_as(_u32p))
+
+    @classmethod
+    def from_seed_file(cls, path) -> "R250":
+        """State = words 7..256 of a LeMonADE seed file (little-endian uint32)."""
+        words = np.frombuffer(open(path, "rb").read(), dtype="<u4")
+        return cls(words[7:257])
+
+    def rand32(self) -> int:
+        return int(lib().oracle_r250_rand32(C.byref(self._s)))
+
+    def substep_draws(self, n_substeps: int, n_colors: int) -> np.ndarray:
+        """Per substep (species, kernel seed) = (rand32 % n_colors, rand32), as CRITICAL.cu:509-510."""
+        out = np.empty(2 * n_substeps, dtype=np.uint32)
+        for k in range(n_substeps):
+            out[2 * k] = self.rand32() % n_colors
+            out[2 * k + 1] = self.rand32()
+        return out
+
+
 class System:
     """A system in oracle form: unfolded corners, CSR bonds, proper coloring."""
 
@@ -126,6 +163,8 @@ class System:
         self.n_colors = int(self.colors.max()) + 1 if self.colors.size else 0
         self.forbidden = standard_bondset() if forbidden is None else np.ascontiguousarray(forbidden, np.uint8)
         self.seed = int(seed)
+        self.draws = None      # optional explicit draws: 2 words per substep from draw_t0 on
+        self.draw_t0 = 0
 
     def _struct(self) -> OracleSystem:
         s = OracleSystem()
@@ -139,6 +178,10 @@ class System:
         s.n_colors = self.n_colors
         s.forbidden = self.forbidden.ctypes.data_as(_u8p)
         s.seed = self.seed & 0xFFFFFFFFFFFFFFFF
+        if self.draws is not None:
+            self.draws = np.ascontiguousarray(self.draws, dtype=np.uint32)
+            s.draws = self.draws.ctypes.data_as(_u32p)
+            s.draw_t0 = int(self.draw_t0)
         return s
 
     def run(self, t0: int, n_substeps: int, threads: int = 0, shuffle_seed: int = 0):

@@ -36,6 +36,32 @@ uint64_t oracle_splitmix64(uint64_t seed, uint64_t t) {
     return z ^ (z >> 31);
 }
 
+void oracle_r250_load(oracle_r250* r, const uint32_t state[250]) {
+    memcpy(r->buf, state, sizeof r->buf);
+    r->index = 0;
+}
+
+uint32_t oracle_r250_rand32(oracle_r250* r) {
+    /* x[n] = x[n-250] ^ x[n-147]: the slot being replaced holds x[n-250] and
+     * the one 103 slots ahead of it (modulo 250) was written 147 draws ago */
+    const uint32_t j = r->index >= 147 ? r->index - 147 : r->index + 103;
+    const uint32_t v = r->buf[r->index] ^ r->buf[j];
+    r->buf[r->index] = v;
+    r->index = r->index == 249 ? 0 : r->index + 1;
+    return v;
+}
+
+/* Species and kernel seed of a substep: the explicit draw list when given,
+ * else the counter-based draw. */
+static void draw_for(const oracle_system* s, uint64_t t, uint32_t* sp, uint32_t* sigma) {
+    if (s->draws) {
+        *sp = s->draws[2 * (t - s->draw_t0)];
+        *sigma = s->draws[2 * (t - s->draw_t0) + 1];
+    } else {
+        oracle_substep_draw(s->seed, t, s->n_colors, sp, sigma);
+    }
+}
+
 void oracle_substep_draw(uint64_t seed, uint64_t t, uint32_t n_colors, uint32_t* species, uint32_t* kernel_seed) {
     const uint64_t r = oracle_splitmix64(seed, t);
     if (species) *species = (uint32_t)(r >> 32) % (n_colors ? n_colors : 1u);
@@ -169,9 +195,18 @@ static void shuffle_order(uint32_t* order, uint32_t n, uint64_t key, uint64_t t)
     }
 }
 
+/* An explicit draw list must cover the substeps and name existing species. */
+static int draws_ok(const oracle_system* s, uint64_t t0, uint64_t n_substeps) {
+    if (!s->draws) return 1;
+    if (t0 < s->draw_t0) return 0;
+    for (uint64_t k = 0; k < n_substeps; ++k)
+        if (s->draws[2 * (t0 + k - s->draw_t0)] >= s->n_colors) return 0;
+    return 1;
+}
+
 int64_t oracle_run(oracle_system* s, uint64_t t0, uint64_t n_substeps, uint32_t* acc_log, int n_threads,
                    uint64_t shuffle_seed) {
-    if (!validate(s)) return -1;
+    if (!validate(s) || !draws_ok(s, t0, n_substeps)) return -1;
     if (s->n == 0 || s->n_colors == 0) {
         if (acc_log) memset(acc_log, 0, sizeof(uint32_t) * n_substeps);
         return 0;
@@ -196,7 +231,7 @@ int64_t oracle_run(oracle_system* s, uint64_t t0, uint64_t n_substeps, uint32_t*
     int64_t total = 0;
     for (uint64_t step = 0; step < n_substeps; ++step) {
         uint32_t sp, sigma;
-        oracle_substep_draw(s->seed, t0 + step, s->n_colors, &sp, &sigma);
+        draw_for(s, t0 + step, &sp, &sigma);
         const uint32_t* mem = members + start[sp];
         const uint32_t ns = start[sp + 1] - start[sp];
         if (shuffle_seed) shuffle_order(order, ns, shuffle_seed, t0 + step);
@@ -275,7 +310,7 @@ static int cubes_overlap(const uint32_t box[3], const int32_t* a, const int32_t*
 }
 
 int64_t oracle_run_naive(oracle_system* s, uint64_t t0, uint64_t n_substeps, uint32_t* acc_log) {
-    if (!validate(s)) return -1;
+    if (!validate(s) || !draws_ok(s, t0, n_substeps)) return -1;
     if (s->n == 0 || s->n_colors == 0) {
         if (acc_log) memset(acc_log, 0, sizeof(uint32_t) * n_substeps);
         return 0;
@@ -288,7 +323,7 @@ int64_t oracle_run_naive(oracle_system* s, uint64_t t0, uint64_t n_substeps, uin
     int64_t total = 0;
     for (uint64_t step = 0; step < n_substeps; ++step) {
         uint32_t sp, sigma;
-        oracle_substep_draw(s->seed, t0 + step, s->n_colors, &sp, &sigma);
+        draw_for(s, t0 + step, &sp, &sigma);
         const uint32_t* mem = members + start[sp];
         const uint32_t ns = start[sp + 1] - start[sp];
         for (uint32_t j = 0; j < ns; ++j) {

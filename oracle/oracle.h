@@ -65,7 +65,26 @@ typedef struct oracle_system {
     uint32_t n_colors;
     const uint8_t* forbidden;   /* 512 entries, index oracle_bond_index */
     uint64_t seed;              /* key of the substep draw */
+    const uint32_t* draws;      /* optional: 2 words per substep (species, kernel seed) from substep draw_t0 on */
+    uint64_t draw_t0;
 } oracle_system;
+
+/* R250 as the reference uses it for the per-substep draws
+ * (CRITICAL.cu:509-510, LeMonADE's RandomNumberGenerators, absent here),
+ * restated from the published generator (Kirkpatrick and Stoll 1981) in
+ * the 250-word ring form of Maier's r250.c: with the loaded words as
+ * x[0..249], the stream is x[n] = x[n-250] XOR x[n-147] (the trinomial
+ * x^250 + x^147 + 1, reciprocal of the 103-lag form).  The seed-file
+ * layout (state = words 7..256 of the file, word 0 = srand seed) is pinned by
+ * the known answers of mainCUDASimulatorGPUScBFM_AB_Type.cpp:137-139 with
+ * tests/resultNormPscBFM.seeds.  Whether LeMonADE adjusts the loaded state
+ * beyond that (outputs 2 and later) is parity unpinned. */
+typedef struct oracle_r250 {
+    uint32_t buf[250];
+    uint32_t index;
+} oracle_r250;
+void oracle_r250_load(oracle_r250* r, const uint32_t state[250]);
+uint32_t oracle_r250_rand32(oracle_r250* r);
 
 /* Excluded volume of front plane (CRITICAL.cu:62-176): nonzero if any of
  * the nine corners p + 2e_d + a e_u + b e_v (a,b in {-1,0,1}) is occupied in

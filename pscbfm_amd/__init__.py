@@ -120,6 +120,10 @@ _SIGNATURES = {
     "pscbfm_check_system": (C.c_int, [_H]),
     "pscbfm_cleanup": (C.c_int, [_H]),
     "pscbfm_set_seed": (C.c_int, [_H, C.c_uint64]),
+    "pscbfm_set_r250_state": (C.c_int, [_H, _u32p]),
+    "pscbfm_r250_from_seed_file": (C.c_int, [_H, C.c_char_p, C.c_uint32]),
+    "pscbfm_r250_next": (C.c_int, [_H, _u32p, C.c_uint64]),
+    "pscbfm_r250_get_state": (C.c_int, [_H, _u32p]),
     "pscbfm_set_lattice_kind": (C.c_int, [_H, C.c_int]),
     "pscbfm_set_uniform_coloring": (C.c_int, [_H, C.c_int]),
     "pscbfm_set_check_on_initialize": (C.c_int, [_H, C.c_int]),
@@ -301,6 +305,29 @@ class UpdaterGPUScBFM_AB_Type:
     # -- extensions -----------------------------------------------------------
     def setSeed(self, seed: int) -> None:
         self._call("pscbfm_set_seed", int(seed) & 0xFFFFFFFFFFFFFFFF)
+
+    def setR250State(self, words: Sequence[int]) -> None:
+        """R250 mode: the substep draws come from this R250 state (250 words in
+        draw order), species = r250 % nColors then seed = r250 (CRITICAL.cu:509-510)."""
+        arr = np.ascontiguousarray(words, dtype=np.uint32)
+        if arr.size != 250:
+            raise ValueError("an R250 state is 250 words")
+        self._call("pscbfm_set_r250_state", _ptr(arr, C.c_uint32))
+
+    def loadR250SeedFile(self, path, discard: int = 0) -> None:
+        """R250 mode from a LeMonADE seed file (state = words 7..256), then
+        `discard` draws (the reference simulator discards 2)."""
+        self._call("pscbfm_r250_from_seed_file", str(path).encode(), int(discard))
+
+    def r250Next(self, n: int) -> np.ndarray:
+        out = np.empty(max(int(n), 1), dtype=np.uint32)
+        self._call("pscbfm_r250_next", _ptr(out, C.c_uint32), int(n))
+        return out[:int(n)]
+
+    def r250State(self) -> np.ndarray:
+        out = np.empty(250, dtype=np.uint32)
+        self._call("pscbfm_r250_get_state", _ptr(out, C.c_uint32))
+        return out
 
     def setLatticeKind(self, kind: int) -> None:
         self._call("pscbfm_set_lattice_kind", int(kind))

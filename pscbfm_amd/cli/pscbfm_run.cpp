@@ -8,10 +8,13 @@
 // Reads the last configuration of the input file, runs MAX_MCS sweeps in
 // blocks of SAVE_INTERVAL and appends a frame to the output file after each
 // block (the header and the initial frame first), as the reference's task
-// loop with AnalyzerWriteBfmFile does.  The species/seed draw is keyed by a
-// 64-bit seed: the first two 32-bit words of the -e file (little-endian), or
-// --seed, or a random one when neither is given (the reference seeds its
-// R250 from the seed file or at random the same way).
+// loop with AnalyzerWriteBfmFile does.  With -e the per-substep species and
+// kernel seed come from R250 seeded from the seed file (state = words 7..256),
+// after the two draws the reference's main takes before its run
+// (mainCUDASimulatorGPUScBFM_AB_Type.cpp:138-139), so the stream the sweep
+// sees is the reference's.  Without -e they come from the counter draw keyed
+// by --seed, or by a random key when neither is given (the reference seeds
+// its R250 at random then).
 #include <getopt.h>
 
 #include <chrono>
@@ -19,7 +22,6 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
-#include <fstream>
 #include <iostream>
 #include <random>
 #include <string>
@@ -32,7 +34,7 @@ void print_help() {
     std::cout << "usage: pscbfm_run [options]\n\n"
                  "Simple simulator for the scBFM with excluded volume and bond check on an MI355X GPU\n\n"
                  "    -e, --seeds <file path>\n"
-                 "        seed file (the first two 32-bit words key the counter-based draws)\n"
+                 "        seed file (LeMonADE layout, >= 257 32-bit words); the substep draws use its R250 stream\n"
                  "    -i, --initial-state <file path>\n"
                  "        (required) BFM file to load the configuration to simulate from (last frame)\n"
                  "    -m, --max-mcs <integer>\n"
@@ -43,7 +45,7 @@ void print_help() {
                  "        frames are appended to this file even if it already exists (default outfile.bfm)\n"
                  "    -g, --gpu <integer>\n"
                  "        GPU to use, 0 .. number of GPUs - 1\n"
-                 "        --seed <integer>      64-bit seed of the draws (instead of -e)\n"
+                 "        --seed <integer>      64-bit key of the counter-based draws (used without -e)\n"
                  "        --lattice bit|u8      occupancy lattice storage (default bit)\n";
 }
 
@@ -100,21 +102,14 @@ int main(int argc, char** argv) {
         return 1;
     }
     if (saveInterval == 0) saveInterval = maxMcs;
-    if (!seedFile.empty()) {
-        std::ifstream f(seedFile, std::ios::binary);
-        unsigned char b[8];
-        if (!f.read(reinterpret_cast<char*>(b), 8)) {
-            std::cerr << "cannot read two 32-bit words from " << seedFile << "\n";
-            return 1;
-        }
-        seed = 0;
-        for (int k = 7; k >= 0; --k) seed = (seed << 8) | b[k];
-        haveSeed = true;
+    if (!seedFile.empty() && haveSeed) {
+        std::cerr << "-e and --seed select different draws; give one of them\n";
+        return 1;
     }
-    if (!haveSeed) seed = (uint64_t(std::random_device{}()) << 32) | std::random_device{}();
+    if (seedFile.empty() && !haveSeed) seed = (uint64_t(std::random_device{}()) << 32) | std::random_device{}();
     std::cerr << "infile          = " << infile << "\noutfile         = " << outfile
-              << "\nmax_mcs         = " << maxMcs << "\nsave_interval   = " << saveInterval
-              << "\nseed            = " << seed << "\n";
+              << "\nmax_mcs         = " << maxMcs << "\nsave_interval   = " << saveInterval << "\n";
+    if (seedFile.empty()) std::cerr << "seed            = " << seed << " (counter-based draws)\n";
 
     const auto t0 = clock::now();
     Handle u;
@@ -123,7 +118,15 @@ int main(int argc, char** argv) {
     if ((rc = pscbfm_set_gpu(u.h, gpu))) return fail(u, "setGpu", rc);
     uint64_t mcs = 0;
     if ((rc = pscbfm_bfm_stage(u.h, infile.c_str(), &mcs))) return fail(u, "reading " + infile, rc);
-    if ((rc = pscbfm_set_seed(u.h, seed))) return fail(u, "setSeed", rc);
+    if (seedFile.empty()) {
+        if ((rc = pscbfm_set_seed(u.h, seed))) return fail(u, "setSeed", rc);
+    } else {
+        std::cerr << "Use seeds from: " << seedFile << "\n";
+        if ((rc = pscbfm_r250_from_seed_file(u.h, seedFile.c_str(), 0))) return fail(u, "reading " + seedFile, rc);
+        uint32_t first[2];
+        if ((rc = pscbfm_r250_next(u.h, first, 2))) return fail(u, "R250", rc);
+        std::cerr << "RNG rand: " << first[0] << "\nRNG rand: " << first[1] / 4294967296.0 << "\n";
+    }
     if ((rc = pscbfm_set_lattice_kind(u.h, lattice))) return fail(u, "setLatticeKind", rc);
     if ((rc = pscbfm_initialize(u.h))) return fail(u, "initialize", rc);
     if ((rc = pscbfm_bfm_write(u.h, outfile.c_str(), mcs, 1))) return fail(u, "write", rc);

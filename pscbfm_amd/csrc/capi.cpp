@@ -226,6 +226,28 @@ int pscbfm_set_seed(pscbfm_updater* h, uint64_t seed) {
     return guarded(h, [&](pscbfm::Updater& u) { u.setSeed(seed); });
 }
 
+int pscbfm_set_r250_state(pscbfm_updater* h, const uint32_t* words) {
+    return guarded(h, [&](pscbfm::Updater& u) { u.setR250State(words); });
+}
+
+int pscbfm_r250_from_seed_file(pscbfm_updater* h, const char* path, uint32_t discard) {
+    return guarded(h, [&](pscbfm::Updater& u) {
+        if (!path) throw std::invalid_argument("[seed file] null path");
+        u.loadR250SeedFile(path, discard);
+    });
+}
+
+int pscbfm_r250_next(pscbfm_updater* h, uint32_t* out, uint64_t n) {
+    return guarded(h, [&](pscbfm::Updater& u) { u.r250Draw(out, n); });
+}
+
+int pscbfm_r250_get_state(const pscbfm_updater* h, uint32_t* words) {
+    return guarded_const(h, [&](const pscbfm::Updater& u) {
+        if (!words) throw std::invalid_argument("[r250State] null output");
+        u.r250State(words);
+    });
+}
+
 int pscbfm_set_lattice_kind(pscbfm_updater* h, int kind) {
     return guarded(h, [&](pscbfm::Updater& u) {
         if (kind < 0) throw std::invalid_argument("[setLatticeKind] unknown kind");
@@ -581,9 +603,17 @@ int pscbfm_batch_footprint(pscbfm_batch* b, const pscbfm_updater* staged, uint64
     });
 }
 
+// Batches and request queues draw every replica from (seed, substep); an
+// updater in R250 mode draws from one sequential stream and cannot join them.
+static void require_counter_draws(const pscbfm::Updater& u) {
+    if (u.r250Mode())
+        throw std::invalid_argument("an updater in R250 mode cannot join a batch or request queue (they use counter draws)");
+}
+
 int pscbfm_batch_add(pscbfm_batch* b, const pscbfm_updater* staged, uint32_t* id) {
     if (!staged) return PSCBFM_NULL_HANDLE;
     return batch_guarded(b, [&](pscbfm::Batch& e) {
+        require_counter_draws(staged->engine);
         const uint32_t r = e.add(staged->engine.buildLayout(), staged->engine.seed());
         if (id) *id = r;
     });
@@ -689,6 +719,7 @@ const char* pscbfm_server_last_error(const pscbfm_server* s) { return s ? s->err
 int pscbfm_server_submit(pscbfm_server* s, const pscbfm_updater* staged, uint64_t n_mcs, uint64_t* job) {
     if (!staged) return PSCBFM_NULL_HANDLE;
     return server_guarded(s, [&](pscbfm::Server& e) {
+        require_counter_draws(staged->engine);
         const uint64_t id = e.submit(staged->engine.staged(), n_mcs);
         if (job) *job = id;
     });

@@ -196,7 +196,10 @@ struct DevState {
     uint64_t logFirst;         // substep index stored in log[0]
     uint64_t logCapacity;      // entries in log
     uint64_t* acceptedByBlock; // kMaxPerformBlocks running totals
-    uint64_t pad_[2];
+    // Explicit draws (R250 mode): substep t reads (species, kernel seed) from
+    // drawTable[2(t - drawBase)], [2(t - drawBase) + 1]; null = counter draw.
+    const uint32_t* drawTable;
+    uint64_t drawBase;
 };
 
 // Workgroup assignment of a batched launch: chunk `chunk` of the `chunks`

@@ -57,6 +57,20 @@ __device__ __forceinline__ gptr<T> global_ptr(T* p) {
     return (gptr<T>)p;
 }
 
+// Species and kernel seed of substep t: entry t - drawBase of the state's
+// explicit draw table (two words, species then seed) when it holds one, else
+// the counter draw of (seed, t).
+__device__ __forceinline__ SubstepDraw draw_at(const SysDesc& d, uint64_t t) {
+    const gptr<DevState> st = global_ptr(d.state);
+    const uint32_t* table = st->drawTable;
+    if (table == nullptr) return substep_draw(d.seed, t, d.nColors);
+    const uint64_t w = global_ptr(reinterpret_cast<const uint64_t*>(table))[t - st->drawBase];
+    SubstepDraw r;
+    r.species = static_cast<uint32_t>(w);
+    r.seed = static_cast<uint32_t>(w >> 32);
+    return r;
+}
+
 // Species record, read word by word through a global pointer.
 __device__ __forceinline__ ColorInfo load_color(const ColorInfo* colors, uint32_t species) {
     const gptr<const uint32_t> w = (gptr<const uint32_t>)global_ptr(colors + species);
@@ -146,7 +160,7 @@ constexpr uint32_t kDirShift = 29;
 template <int LAT, int SLOTS, int MAXDEG>
 __device__ __forceinline__ void check_body(const SysDesc& d, uint64_t t, uint32_t lb, gptr<uint64_t> list,
                                            gptr<uint32_t> count, uint32_t* sForbidden, uint32_t* sCount) {
-    const SubstepDraw draw = substep_draw(d.seed, t, d.nColors);
+    const SubstepDraw draw = draw_at(d, t);
     const ColorInfo ci = load_color(d.colors, draw.species);
     const gptr<uint32_t> proposal = global_ptr(d.proposal);
 
@@ -242,7 +256,7 @@ template <int LAT>
 __device__ __forceinline__ void perform_body(const SysDesc& d, uint64_t t, uint32_t lb, uint32_t nb,
                                              gptr<const uint64_t> list, uint32_t count, gptr<uint64_t> accSlot,
                                              uint32_t* sSum) {
-    const SubstepDraw draw = substep_draw(d.seed, t, d.nColors);
+    const SubstepDraw draw = draw_at(d, t);
     const ColorInfo ci = load_color(d.colors, draw.species);
     const uint32_t B = blockDim.x;
 

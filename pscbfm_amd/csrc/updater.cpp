@@ -10,6 +10,9 @@
 
 namespace pscbfm {
 
+// Substeps of explicit draws kept on the device at once in R250 mode (32 MiB).
+constexpr uint64_t kMaxDrawSubsteps = uint64_t(1) << 22;
+
 namespace {
 
 bool is_pow2(uint32_t v) { return v != 0 && (v & (v - 1)) == 0; }
@@ -278,6 +281,9 @@ void Updater::freeDevice() {
         graphExec_ = nullptr;
     }
     dev_.release();
+    if (drawTable_) (void)hipFree(drawTable_);
+    drawTable_ = nullptr;
+    drawCapacity_ = 0;
     if (stream_) {
         (void)hipStreamDestroy(stream_);
         stream_ = nullptr;
@@ -398,18 +404,83 @@ void Updater::run(int32_t nMCS, uint32_t* log, bool profile, bool useGraph) {
     const uint32_t nc = nColors();
     if (nc == 0 || n_ == 0) return;
     const uint64_t S = uint64_t(nMCS) * nc;
+    // R250 mode keeps the draw table of one chunk of whole sweeps on the
+    // device at a time (at most kMaxDrawSubsteps substeps, 8 bytes each).
+    const uint64_t chunk = r250Mode_ ? std::max<uint64_t>(1u, kMaxDrawSubsteps / nc) * nc : S;
+    RunStats total{};
+    for (uint64_t done = 0; done < S; done += chunk) {
+        const uint64_t part = std::min(chunk, S - done);
+        runChunk(part, log ? log + done : nullptr, profile, useGraph);
+        total.gpuMilliseconds += lastRun_.gpuMilliseconds;
+        total.substeps += lastRun_.substeps;
+        total.attempted += lastRun_.attempted;
+        total.bondTerms += lastRun_.bondTerms;
+        total.checkMilliseconds += lastRun_.checkMilliseconds;
+        total.performMilliseconds += lastRun_.performMilliseconds;
+        total.checkLaunches += lastRun_.checkLaunches;
+        total.performLaunches += lastRun_.performLaunches;
+    }
+    lastRun_ = total;
+    syncFromDevice();
+    const DevState st = dev_.readState(stream_);
+    std::vector<uint64_t> perBlock(performGridMax_);
+    hip_check(hipMemcpy(perBlock.data(), st.acceptedByBlock, perBlock.size() * sizeof(uint64_t), hipMemcpyDeviceToHost),
+              "accepted D2H");
+    uint64_t acc = 0;
+    for (uint64_t v : perBlock) acc += v;
+    lastRun_.accepted = acc - acceptedTotal_;
+    acceptedTotal_ = acc;
+    if (st.substep != substep_) throw std::runtime_error("device substep counter out of step with the host");
+}
+
+// S substeps from substep_ on; fills lastRun_ except `accepted`.
+void Updater::runChunk(uint64_t S, uint32_t* log, bool profile, bool useGraph) {
+    lastRun_ = RunStats{};
+    const uint32_t nc = nColors();
+
+    // Species and kernel seed of every substep, as the kernels will draw them.
+    std::vector<uint32_t> species(S);
+    std::vector<uint32_t> draws;
+    if (r250Mode_) {
+        // CRITICAL.cu:509-510: species first, then the kernel seed
+        draws.resize(2 * S);
+        for (uint64_t k = 0; k < S; ++k) {
+            draws[2 * k] = r250_.next() % nc;
+            draws[2 * k + 1] = r250_.next();
+            species[k] = draws[2 * k];
+        }
+        if (drawCapacity_ < S) {
+            if (drawTable_) (void)hipFree(drawTable_);
+            drawTable_ = nullptr;
+            drawCapacity_ = 0;
+            hip_check(hipMalloc(&drawTable_, S * 8), "hipMalloc (draw table)");
+            drawCapacity_ = S;
+        }
+        hip_check(hipMemcpyAsync(drawTable_, draws.data(), S * 8, hipMemcpyHostToDevice, stream_), "draw table H2D");
+    } else {
+        for (uint64_t k = 0; k < S; ++k) species[k] = substep_draw(seed_, substep_ + k, nc).species;
+    }
 
     uint32_t* dLog = nullptr;
     if (log) {
         hip_check(hipMalloc(&dLog, S * 4), "hipMalloc (log)");
         hip_check(launch_fill_u32(dLog, S, 0u, stream_), "fill log");
-        DevState patch{};
-        hip_check(hipMemcpyAsync(&patch, dev_.desc.state, sizeof patch, hipMemcpyDeviceToHost, stream_), "state D2H");
-        hip_check(hipStreamSynchronize(stream_), "sync");
-        patch.log = dLog;
-        patch.logFirst = substep_;
-        patch.logCapacity = S;
-        hip_check(hipMemcpyAsync(dev_.desc.state, &patch, sizeof patch, hipMemcpyHostToDevice, stream_), "state H2D");
+    }
+    const bool patched = log != nullptr || r250Mode_;
+    if (patched) {
+        DevState patch = dev_.readState(stream_);
+        if (log) {
+            patch.log = dLog;
+            patch.logFirst = substep_;
+            patch.logCapacity = S;
+        }
+        if (r250Mode_) {
+            // the table covers substeps [substep_, substep_ + S); none beyond runs
+            patch.drawTable = drawTable_;
+            patch.drawBase = substep_;
+            patch.stopAt = substep_ + S;
+        }
+        dev_.writeState(patch, stream_);
     }
 
     hipEvent_t e0, e1;
@@ -437,9 +508,8 @@ void Updater::run(int32_t nMCS, uint32_t* log, bool profile, bool useGraph) {
                 done += graphSubsteps_;
             }
         }
-        const uint64_t rem = S - done;
-        // Eager tail in blocks of at most 2^31 substeps per advance.
-        uint64_t left = rem;
+        // Eager tail in blocks of at most 2^20 substeps per advance.
+        uint64_t left = S - done;
         while (left > 0) {
             const uint32_t blk = uint32_t(std::min<uint64_t>(left, 1u << 20));
             for (uint32_t k = 0; k < blk; ++k) {
@@ -470,45 +540,53 @@ void Updater::run(int32_t nMCS, uint32_t* log, bool profile, bool useGraph) {
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
 
-    // Attempts and bond terms follow from the host replica of the draws.
+    // Attempts and bond terms follow from the host copy of the draws: every
+    // member of the drawn species is attempted, with all its bonds.
+    std::vector<uint64_t> deg(nc, 0);
+    for (uint32_t i = 0; i < n_; ++i) deg[layout_.color[i]] += degree_[i];
     for (uint64_t k = 0; k < S; ++k) {
-        const SubstepDraw d = substep_draw(seed_, substep_ + k, nc);
-        const ColorInfo& ci = layout_.colors[d.species];
-        lastRun_.attempted += ci.count;
-    }
-    {
-        // Bond terms per species: total degree of its members.
-        std::vector<uint64_t> deg(nc, 0);
-        for (uint32_t i = 0; i < n_; ++i) deg[layout_.color[i]] += degree_[i];
-        for (uint64_t k = 0; k < S; ++k) lastRun_.bondTerms += deg[substep_draw(seed_, substep_ + k, nc).species];
+        lastRun_.attempted += layout_.colors[species[k]].count;
+        lastRun_.bondTerms += deg[species[k]];
     }
     lastRun_.substeps = S;
     substep_ += S;
 
-    if (log) {
-        hip_check(hipMemcpy(log, dLog, S * 4, hipMemcpyDeviceToHost), "log D2H");
-        DevState patch{};
-        hip_check(hipMemcpy(&patch, dev_.desc.state, sizeof patch, hipMemcpyDeviceToHost), "state D2H");
+    if (log) hip_check(hipMemcpy(log, dLog, S * 4, hipMemcpyDeviceToHost), "log D2H");
+    if (patched) {
+        DevState patch = dev_.readState(stream_);
         patch.log = nullptr;
         patch.logCapacity = 0;
-        hip_check(hipMemcpy(dev_.desc.state, &patch, sizeof patch, hipMemcpyHostToDevice), "state H2D");
-        (void)hipFree(dLog);
+        patch.drawTable = nullptr;
+        patch.drawBase = 0;
+        patch.stopAt = std::numeric_limits<uint64_t>::max();
+        dev_.writeState(patch, stream_);
     }
-    syncFromDevice();
-    lastRun_.accepted = 0;
-    {
-        DevState st{};
-        hip_check(hipMemcpy(&st, dev_.desc.state, sizeof st, hipMemcpyDeviceToHost), "state D2H");
-        std::vector<uint64_t> perBlock(performGridMax_);
-        hip_check(hipMemcpy(perBlock.data(), st.acceptedByBlock, perBlock.size() * sizeof(uint64_t),
-                            hipMemcpyDeviceToHost),
-                  "accepted D2H");
-        uint64_t total = 0;
-        for (uint64_t v : perBlock) total += v;
-        lastRun_.accepted = total - acceptedTotal_;
-        acceptedTotal_ = total;
-        if (st.substep != substep_) throw std::runtime_error("device substep counter out of step with the host");
+    if (dLog) (void)hipFree(dLog);
+}
+
+void Updater::setR250State(const uint32_t* words250) {
+    if (!words250) throw std::invalid_argument("[setR250State] null state");
+    r250_.load(words250);
+    r250Mode_ = true;
+}
+
+void Updater::loadR250SeedFile(const std::string& path, uint32_t discard) {
+    const std::vector<uint32_t> w = read_r250_seed_file(path);
+    setR250State(w.data());
+    for (uint32_t k = 0; k < discard; ++k) (void)r250_.next();
+}
+
+void Updater::r250Draw(uint32_t* out, uint64_t n) {
+    if (!r250Mode_) throw std::runtime_error("[r250Draw] no R250 state was set");
+    for (uint64_t k = 0; k < n; ++k) {
+        const uint32_t v = r250_.next();
+        if (out) out[k] = v;
     }
+}
+
+void Updater::r250State(uint32_t* words250) const {
+    if (!r250Mode_) throw std::runtime_error("[r250State] no R250 state was set");
+    r250_.state(words250);
 }
 
 // CRITICAL.cu:414-698 (synchronous; returns after copy-back and unsort)

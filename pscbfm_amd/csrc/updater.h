@@ -13,6 +13,7 @@
 #include "device_system.h"
 #include "kernels.h"
 #include "layout.h"
+#include "r250.h"
 
 namespace pscbfm {
 
@@ -78,6 +79,14 @@ public:
     void setGraphChunk(uint32_t substeps) { graphChunk_ = substeps; }
     // Pass-2 threads per workgroup (0 = the pass-1 block).
     void setPerformBlock(uint32_t block);
+    // R250 mode: from the next run on, every substep draws its species
+    // (r250 % nColors) and kernel seed (r250) from this generator in the
+    // reference's order, instead of the counter draw of (seed, substep).
+    void setR250State(const uint32_t* words250);
+    void loadR250SeedFile(const std::string& path, uint32_t discard);
+    void r250Draw(uint32_t* out, uint64_t n);
+    void r250State(uint32_t* words250) const;
+    bool r250Mode() const { return r250Mode_; }
     void setMonomerCoordinatesBulk(const int32_t* xyz, uint32_t n);
     void setConnectivityBulk(const uint32_t* pairs, uint64_t nPairs);
     void getPositionsBulk(int32_t* xyz, uint32_t n) const;
@@ -118,6 +127,7 @@ private:
     void requireMonomers(const char* who) const;
     void requireIndex(uint32_t i, const char* who) const;
     void freeDevice();
+    void runChunk(uint64_t S, uint32_t* log, bool profile, bool useGraph);
     void captureGraph();
     void syncFromDevice();
     void applyLaunchShape();
@@ -136,6 +146,8 @@ private:
     std::vector<uint8_t> colorOverride_;
     bool hasColorOverride_ = false;
     uint64_t seed_ = 0;
+    bool r250Mode_ = false;
+    R250 r250_;
     uint32_t latticeKind_ = kLatticeBit;
     bool uniformColors_ = true;
     bool checkOnInit_ = true;
@@ -154,6 +166,8 @@ private:
     uint64_t substep_ = 0;
     uint64_t acceptedTotal_ = 0;
     uint64_t cells_ = 0;
+    uint32_t* drawTable_ = nullptr;  // R250 mode: 2 words per substep of one run chunk
+    uint64_t drawCapacity_ = 0;      // substeps drawTable_ holds
     RunStats lastRun_;
 };
 

@@ -163,9 +163,20 @@ CPP_DRIVER = r"""
 #include <cstdio>
 #include <stdexcept>
 #include "UpdaterGPUScBFM_AB_Type.h"
-int main() {
+int main(int argc, char** argv) {
     int fails = 0;
     UpdaterGPUScBFM_AB_Type u;
+    if (argc > 1) {   // argv[1]: a seed file; the loaded state is its words 7..256
+        u.loadR250SeedFile(argv[1], 0);
+        uint32_t st[250];
+        u.getR250State(st);
+        std::FILE* f = std::fopen(argv[1], "rb");
+        uint32_t w[257];
+        if (!f || std::fread(w, 4, 257, f) != 257) fails++;
+        if (f) std::fclose(f);
+        for (int k = 0; k < 250; ++k) fails += st[k] != w[7 + k];
+        try { u.setR250State(nullptr); fails++; } catch (const std::invalid_argument&) {}
+    }
     try { u.setPeriodicity(true, false, true); fails++; } catch (const std::invalid_argument&) {}
     u.setNrOfAllMonomers(2);
     try { u.setNrOfAllMonomers(2); fails++; } catch (const std::runtime_error&) {}
@@ -194,7 +205,7 @@ def test_cpp_dropin_header_compiles_links_and_throws(tmp_path):
     libdir = P.library_path.parent
     subprocess.run(["g++", "-std=c++11", "-O1", "-I", str(ROOT / "include"), str(src), "-o", str(exe),
                     "-L", str(libdir), "-lpscbfm", f"-Wl,-rpath,{libdir}"], check=True)
-    res = subprocess.run([str(exe)], capture_output=True, text=True)
+    res = subprocess.run([str(exe), str(GOLDEN / "resultNormPscBFM.seeds")], capture_output=True, text=True)
     assert res.returncode == 0, res.stdout + res.stderr
     assert "fails=0" in res.stdout
 
@@ -371,3 +382,46 @@ def test_lemonade_bond_ids_match_the_reference_fixture():
     from pscbfm_amd.bfm import lemonade_bond_ids
     ref = read_bfm(GOLDEN / "LinearChain_l1024_n64.bfm.gz").bondset
     assert {k: tuple(v) for k, v in ref.items()} == lemonade_bond_ids()
+
+
+# ---- R250 mode (host side; no kernel runs) -----------------------------------------
+def test_library_r250_matches_known_answers_and_oracle():
+    seeds = GOLDEN / "resultNormPscBFM.seeds"
+    u = P.UpdaterGPUScBFM_AB_Type()
+    u.loadR250SeedFile(seeds, 0)
+    first = u.r250Next(2)
+    # mainCUDASimulatorGPUScBFM_AB_Type.cpp:138-139
+    assert int(first[0]) == 4223731124 and f"{first[1] / 2**32:.6f}" == "0.803876"
+    o = oracle.R250.from_seed_file(seeds)
+    o.rand32(), o.rand32()
+    assert u.r250Next(1000).tolist() == [o.rand32() for _ in range(1000)]
+
+
+def test_library_r250_state_round_trip_and_discard():
+    seeds = GOLDEN / "resultNormPscBFM.seeds"
+    a = P.UpdaterGPUScBFM_AB_Type()
+    a.loadR250SeedFile(seeds, 2)
+    a.r250Next(377)                      # leave the ring mid-way
+    b = P.UpdaterGPUScBFM_AB_Type()
+    b.setR250State(a.r250State())        # continues the same stream
+    assert a.r250Next(600).tolist() == b.r250Next(600).tolist()
+    c = P.UpdaterGPUScBFM_AB_Type()
+    c.loadR250SeedFile(seeds, 2)
+    d = P.UpdaterGPUScBFM_AB_Type()
+    d.loadR250SeedFile(seeds, 0)
+    d.r250Next(2)
+    assert c.r250Next(50).tolist() == d.r250Next(50).tolist()
+
+
+def test_library_r250_errors(tmp_path):
+    u = P.UpdaterGPUScBFM_AB_Type()
+    with pytest.raises(Exception, match="R250"):
+        u.r250Next(1)                    # no state yet
+    short = tmp_path / "short.seeds"
+    short.write_bytes(b"\0" * (4 * 256))
+    with pytest.raises(Exception, match="fewer than 257"):
+        u.loadR250SeedFile(short)
+    with pytest.raises(Exception, match="cannot open"):
+        u.loadR250SeedFile(tmp_path / "missing.seeds")
+    with pytest.raises(ValueError):
+        u.setR250State(np.zeros(10, dtype=np.uint32))

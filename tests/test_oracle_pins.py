@@ -245,3 +245,60 @@ def test_sequential_baseline_statistics_match_colored_scheme():
         return float((v.astype(np.int64) ** 2).sum(1).mean())
     assert abs(mean_b2(a.xyz) - mean_b2(b.xyz)) / mean_b2(b.xyz) < 0.05
     assert a.check() == 0 and b.check() == 0
+
+
+# ---- R250 (the reference's per-substep draws) ---------------------------------------
+def test_r250_hits_the_reference_known_answers():
+    # mainCUDASimulatorGPUScBFM_AB_Type.cpp:137-139, seeded from tests/resultNormPscBFM.seeds
+    # (testCorrectness.sh:208): std::rand() 764080779, r250_rand32() 4223731124, r250_drand() 0.803876
+    import ctypes
+    words = np.frombuffer((GOLDEN / "resultNormPscBFM.seeds").read_bytes(), dtype="<u4")
+    libc = ctypes.CDLL("libc.so.6")
+    libc.srand(ctypes.c_uint(int(words[0])))
+    assert libc.rand() == 764080779
+    r = oracle.R250.from_seed_file(GOLDEN / "resultNormPscBFM.seeds")
+    assert r.rand32() == 4223731124
+    assert f"{r.rand32() / 2**32:.6f}" == "0.803876"
+
+
+def test_r250_recurrence_closed_form():
+    # the buffer form draws x[n] = x[n-250] ^ x[n-147] (x^250 + x^147 + 1, the reciprocal of the
+    # 103-lag trinomial), with the 250 state words as x[0..249]; checked on 2000 outputs
+    state = np.random.default_rng(3).integers(0, 2**32, 250, dtype=np.uint64).astype(np.uint32)
+    r = oracle.R250(state)
+    x = list(int(v) for v in state) + [r.rand32() for _ in range(2000)]
+    for n in range(250, len(x)):
+        assert x[n] == x[n - 250] ^ x[n - 147]
+
+
+def test_oracle_explicit_draws_equal_counter_draws():
+    s = chains((32, 32, 32), 20, 16, seed=7)
+    start, ids = s.csr()
+    col = P.color_graph(s.n, start, ids)
+    nc = int(col.max()) + 1
+    a = oracle_for(s, col, 42)
+    ta, loga = a.run(0, 40)
+    draws = np.array([oracle.substep_draw(42, t, nc) for t in range(40)], dtype=np.uint32).ravel()
+    b = oracle_for(s, col, 999)  # the key is ignored when draws are given
+    b.draws, b.draw_t0 = draws, 0
+    tb, logb = b.run(0, 40)
+    assert ta == tb and np.array_equal(loga, logb) and np.array_equal(a.xyz, b.xyz)
+
+
+def test_oracle_r250_draws_are_order_independent_and_match_naive():
+    s = chains((16, 16, 16), 6, 10, seed=3)
+    start, ids = s.csr()
+    col = P.color_graph(s.n, start, ids)
+    draws = oracle.R250.from_seed_file(GOLDEN / "resultNormPscBFM.seeds").substep_draws(30, int(col.max()) + 1)
+    res = []
+    for mode in ("fast", "shuffled", "naive"):
+        o = oracle_for(s, col, 0)
+        o.draws = draws
+        if mode == "fast":
+            t, log = o.run(0, 30)
+        elif mode == "shuffled":
+            t, log = o.run(0, 30, shuffle_seed=11)
+        else:
+            t, log = o.run_naive(0, 30)
+        res.append((t, log.tolist(), o.xyz.tolist()))
+    assert res[0] == res[1] == res[2]
