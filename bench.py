@@ -102,6 +102,56 @@ def cpu_baseline(u, box, bonds, colors, seed, target_seconds: float):
     }, parity, mcs
 
 
+def cpu_single_core(u, box, bonds, colors, seed, target_seconds: float) -> dict:
+    """The two single-core CPU figures beside the all-cores oracle: the colored
+    oracle on one thread, and the sequential LeMonADE-style simulator (random
+    monomer, random direction, apply at once; N attempts per MCS) on one
+    thread.  Each runs whole MCS of the current melt state for about
+    `target_seconds`."""
+    import oracle
+
+    n = colors.size
+    xyz0 = u.getPositions(n)
+    t0, _ = u.counters()
+    n_colors = int(colors.max()) + 1
+    a = np.concatenate([bonds[:, 0], bonds[:, 1]]).astype(np.int64)
+    b = np.concatenate([bonds[:, 1], bonds[:, 0]]).astype(np.int64)
+    order = np.argsort(a, kind="stable")
+    a, b = a[order], b[order]
+    start = np.zeros(n + 1, dtype=np.int64)
+    np.add.at(start, a + 1, 1)
+    start = np.cumsum(start).astype(np.uint32)
+    sizes = np.bincount(colors, minlength=n_colors)
+    out = {}
+    # colored oracle, one thread: one substep to calibrate, then whole MCS
+    probe = oracle.System(box, xyz0, start, b, colors, seed=seed)
+    tic = time.perf_counter()
+    probe.run(t0, 1, threads=1)
+    per_substep = time.perf_counter() - tic
+    mcs = max(1, int(target_seconds / max(per_substep * n_colors, 1e-3)))
+    sysm = oracle.System(box, xyz0, start, b, colors, seed=seed)
+    tic = time.perf_counter()
+    sysm.run(t0, mcs * n_colors, threads=1)
+    dt = time.perf_counter() - tic
+    attempted = sum(int(sizes[P.substep_draw(seed, t0 + k, n_colors)[0]]) for k in range(mcs * n_colors))
+    out["oracle_1_core"] = {"value": attempted / dt, "unit": "attempted monomer moves/s", "cores": 1,
+                            "kind": "port", "sample": f"{mcs} MCS ({attempted} attempts), {dt:.1f} s"}
+    # sequential simulator, one thread
+    seq = oracle.System(box, xyz0, start, b, colors, seed=seed)
+    tic = time.perf_counter()
+    seq.sequential(seed, 1)
+    per_mcs = time.perf_counter() - tic
+    mcs = max(1, int(target_seconds / max(per_mcs, 1e-3)))
+    seq = oracle.System(box, xyz0, start, b, colors, seed=seed)
+    tic = time.perf_counter()
+    acc = seq.sequential(seed, mcs)
+    dt = time.perf_counter() - tic
+    out["sequential_1_core"] = {"value": mcs * n / dt, "unit": "attempted monomer moves/s", "cores": 1,
+                                "kind": "port", "acceptance": acc / (mcs * n),
+                                "sample": f"{mcs} MCS ({mcs * n} attempts), {dt:.1f} s"}
+    return out
+
+
 def network_section(seed: int, mcs: int, kind: int, c_l: float) -> dict:
     """Star network: 12800 4-arm stars of arm length 29 (1,497,600 monomers)
     with arm ends crosslinked to other stars, on a 512x256x256 periodic box
@@ -253,6 +303,8 @@ def main() -> int:
     ap.add_argument("--profile-mcs", type=int, default=20)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-single-seconds", type=float, default=5.0,
+                    help="seconds for each single-core CPU figure (0 = skip them)")
     ap.add_argument("--replicas", type=int, default=1024, help="systems of the batched comparison")
     ap.add_argument("--replica-mcs", type=int, default=500)
     ap.add_argument("--tune-sample", type=int, default=64)
@@ -394,6 +446,12 @@ def main() -> int:
         out["cpu_baseline"] = cpu
         out["parity_vs_oracle"] = f"{'bit-exact' if parity else 'MISMATCH'} over {mcs} MCS"
         log(f"CPU {cpu['value'] / 1e6:.1f} M attempts/s on {cpu['cores']} threads; parity {parity}")
+        if args.cpu_single_seconds > 0:
+            one = cpu_single_core(u, box, bonds, colors, args.seed, args.cpu_single_seconds)
+            out["cpu_single_core"] = one
+            log(f"CPU one core: oracle {one['oracle_1_core']['value'] / 1e6:.1f} M attempts/s, sequential "
+                f"{one['sequential_1_core']['value'] / 1e6:.1f} M attempts/s "
+                f"(acceptance {one['sequential_1_core']['acceptance']:.3f})")
     u.cleanup()
     if not args.no_network and args.network_mcs > 0:
         out["network"] = network_section(args.seed, args.network_mcs, kind, c_l)

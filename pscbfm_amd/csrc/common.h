@@ -12,6 +12,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <cstddef>
 #include <cstdint>
 
 #if defined(__HIPCC__)
@@ -190,17 +191,21 @@ constexpr uint32_t kMaxPerformBlocks = 1u << 16;
 // Mutable per-system state that lives on the device so that a captured
 // graph can replay any number of substeps without host involvement.
 struct DevState {
+    // The first 32 bytes are what every sweep launch reads at its start, in
+    // one batch: the substep to serve, the stop point and the draw source.
     uint64_t substep;          // global substep counter t of the next substep
     uint64_t stopAt;           // substeps >= stopAt are skipped (batched runs)
-    uint32_t* log;             // optional per-substep accepted counts
-    uint64_t logFirst;         // substep index stored in log[0]
-    uint64_t logCapacity;      // entries in log
-    uint64_t* acceptedByBlock; // kMaxPerformBlocks running totals
     // Explicit draws (R250 mode): substep t reads (species, kernel seed) from
     // drawTable[2(t - drawBase)], [2(t - drawBase) + 1]; null = counter draw.
     const uint32_t* drawTable;
     uint64_t drawBase;
+    uint32_t* log;             // optional per-substep accepted counts
+    uint64_t logFirst;         // substep index stored in log[0]
+    uint64_t logCapacity;      // entries in log
+    uint64_t* acceptedByBlock; // kMaxPerformBlocks running totals
 };
+static_assert(sizeof(DevState) == 64, "DevState is one 64-byte line");
+static_assert(offsetof(DevState, drawBase) == 24, "the launch words are the first 32 bytes");
 
 // Workgroup assignment of a batched launch: chunk `chunk` of the `chunks`
 // workgroups that cover replica `replica` (kNoReplica: idle workgroup).

@@ -57,18 +57,50 @@ __device__ __forceinline__ gptr<T> global_ptr(T* p) {
     return (gptr<T>)p;
 }
 
-// Species and kernel seed of substep t: entry t - drawBase of the state's
-// explicit draw table (two words, species then seed) when it holds one, else
-// the counter draw of (seed, t).
-__device__ __forceinline__ SubstepDraw draw_at(const SysDesc& d, uint64_t t) {
-    const gptr<DevState> st = global_ptr(d.state);
-    const uint32_t* table = st->drawTable;
-    if (table == nullptr) return substep_draw(d.seed, t, d.nColors);
-    const uint64_t w = global_ptr(reinterpret_cast<const uint64_t*>(table))[t - st->drawBase];
-    SubstepDraw r;
-    r.species = static_cast<uint32_t>(w);
-    r.seed = static_cast<uint32_t>(w >> 32);
-    return r;
+// What a sweep launch reads from its system's state, taken as four 16-byte
+// loads of the whole 64-byte record before any branch, so that the start of
+// every workgroup waits for one round trip: the substep it serves (device
+// counter + k), the stop point, the draw source and the per-substep log and
+// accepted-count pointers.
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
+struct Launch {
+    uint64_t t;
+    SubstepDraw draw;
+    uint32_t* log;
+    uint64_t logFirst, logCapacity;
+    uint64_t* acceptedByBlock;
+};
+
+__device__ __forceinline__ uint64_t join64(uint32_t lo, uint32_t hi) { return uint64_t(lo) | (uint64_t(hi) << 32); }
+
+// False when substep t is at or past the stop point.  Else fills `out`; the
+// draw is entry t - drawBase of the explicit table when the state holds one,
+// else the counter draw of (seed, t).
+__device__ __forceinline__ bool begin_substep(const SysDesc& d, uint32_t k, Launch& out) {
+    const gptr<const u32x4> w = (gptr<const u32x4>)global_ptr(d.state);
+    const u32x4 w0 = w[0], w1 = w[1], w2 = w[2], w3 = w[3];  // DevState field order
+    const uint64_t t = join64(w0.x, w0.y) + k;
+    // A state whose draw table, log or accepted counts are not aligned to
+    // their entries is not served.  Testing them here, with the stop point,
+    // also keeps all four loads ahead of the branch as one batch.
+    const uint32_t misaligned = ((w1.x | w3.z) & 7u) | (w2.x & 3u);
+    if (t >= join64(w0.z, w0.w) || misaligned != 0u) return false;
+    out.t = t;
+    const uint64_t table = join64(w1.x, w1.y);
+    if (table == 0u) {
+        out.draw = substep_draw(d.seed, t, d.nColors);
+    } else {
+        // every lane reads the same entry: keep the draw in scalar registers
+        const uint64_t e = global_ptr(reinterpret_cast<const uint64_t*>(uintptr_t(table)))[t - join64(w1.z, w1.w)];
+        out.draw.species = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(e));
+        out.draw.seed = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(e >> 32));
+    }
+    out.log = reinterpret_cast<uint32_t*>(uintptr_t(join64(w2.x, w2.y)));
+    out.logFirst = join64(w2.z, w2.w);
+    out.logCapacity = join64(w3.x, w3.y);
+    out.acceptedByBlock = reinterpret_cast<uint64_t*>(uintptr_t(join64(w3.z, w3.w)));
+    return true;
 }
 
 // Species record, read word by word through a global pointer.
@@ -158,9 +190,9 @@ constexpr uint32_t kDirShift = 29;
 // Pass 1 of substep t for logical workgroup lb of one system.  The
 // survivors go to `list` (capacity SLOTS * blockDim), their number to *count.
 template <int LAT, int SLOTS, int MAXDEG>
-__device__ __forceinline__ void check_body(const SysDesc& d, uint64_t t, uint32_t lb, gptr<uint64_t> list,
-                                           gptr<uint32_t> count, uint32_t* sForbidden, uint32_t* sCount) {
-    const SubstepDraw draw = draw_at(d, t);
+__device__ __forceinline__ void check_body(const SysDesc& d, uint64_t t, SubstepDraw draw, uint32_t lb,
+                                           gptr<uint64_t> list, gptr<uint32_t> count, uint32_t* sForbidden,
+                                           uint32_t* sCount) {
     const ColorInfo ci = load_color(d.colors, draw.species);
     const gptr<uint32_t> proposal = global_ptr(d.proposal);
 
@@ -253,10 +285,11 @@ __device__ __forceinline__ uint32_t block_sum(uint32_t v, uint32_t* scratch) {
 // Pass 2 of substep t over one pass-1 list.  The accepted count is added to
 // *accSlot, an entry only this workgroup writes.
 template <int LAT>
-__device__ __forceinline__ void perform_body(const SysDesc& d, uint64_t t, uint32_t lb, uint32_t nb,
+__device__ __forceinline__ void perform_body(const SysDesc& d, const Launch& L, uint32_t lb, uint32_t nb,
                                              gptr<const uint64_t> list, uint32_t count, gptr<uint64_t> accSlot,
                                              uint32_t* sSum) {
-    const SubstepDraw draw = draw_at(d, t);
+    const uint64_t t = L.t;
+    const SubstepDraw draw = L.draw;
     const ColorInfo ci = load_color(d.colors, draw.species);
     const uint32_t B = blockDim.x;
 
@@ -318,11 +351,10 @@ __device__ __forceinline__ void perform_body(const SysDesc& d, uint64_t t, uint3
     const uint32_t total = block_sum(accepted, sSum);
     if (threadIdx.x == 0 && total != 0u) {
         *accSlot += total;
-        const gptr<DevState> st = global_ptr(d.state);
-        if (st->log != nullptr) {
-            const uint64_t idx = t - st->logFirst;
-            if (idx < st->logCapacity)
-                __hip_atomic_fetch_add(&global_ptr(st->log)[idx], total, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (L.log != nullptr) {
+            const uint64_t idx = t - L.logFirst;
+            if (idx < L.logCapacity)
+                __hip_atomic_fetch_add(&global_ptr(L.log)[idx], total, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
     }
 }
@@ -332,11 +364,10 @@ template <int LAT, int SLOTS, int MAXDEG>
 __global__ __launch_bounds__(256) void k_check(SysDesc d, uint32_t k) {
     __shared__ uint32_t sForbidden[16];
     __shared__ uint32_t sCount;
-    const gptr<DevState> st = global_ptr(d.state);
-    const uint64_t t = st->substep + k;
-    if (t >= st->stopAt) return;
+    Launch L;
+    if (!begin_substep(d, k, L)) return;
     const uint32_t lb = xcd_block();
-    check_body<LAT, SLOTS, MAXDEG>(d, t, lb,
+    check_body<LAT, SLOTS, MAXDEG>(d, L.t, L.draw, lb,
                                    global_ptr(reinterpret_cast<uint64_t*>(d.list)) + size_t(lb) * (SLOTS * blockDim.x),
                                    global_ptr(d.listCount) + lb, sForbidden, &sCount);
 }
@@ -344,12 +375,11 @@ __global__ __launch_bounds__(256) void k_check(SysDesc d, uint32_t k) {
 template <int LAT>
 __global__ __launch_bounds__(256) void k_perform(SysDesc d, uint32_t k, uint32_t listStride) {
     __shared__ uint32_t sSum[4];
-    const gptr<DevState> st = global_ptr(d.state);
-    const uint64_t t = st->substep + k;
-    if (t >= st->stopAt) return;
+    Launch L;
+    if (!begin_substep(d, k, L)) return;
     const uint32_t lb = xcd_block();  // same numbering as pass 1: list lb
-    perform_body<LAT>(d, t, lb, gridDim.x, global_ptr(reinterpret_cast<const uint64_t*>(d.list)) + size_t(lb) * listStride,
-                      global_ptr(d.listCount)[lb], global_ptr(st->acceptedByBlock) + blockIdx.x, sSum);
+    perform_body<LAT>(d, L, lb, gridDim.x, global_ptr(reinterpret_cast<const uint64_t*>(d.list)) + size_t(lb) * listStride,
+                      global_ptr(d.listCount)[lb], global_ptr(L.acceptedByBlock) + blockIdx.x, sSum);
 }
 
 // ---- batch of replicas ---------------------------------------------------------------
@@ -363,10 +393,9 @@ __global__ __launch_bounds__(256) void k_check_batch(const SysDesc* __restrict__
     const Tile tile = tiles[blockIdx.x];
     if (tile.replica == kNoReplica) return;
     const SysDesc& d = descs[tile.replica];
-    const gptr<DevState> st = global_ptr(d.state);
-    const uint64_t t = st->substep + k;
-    if (t >= st->stopAt) return;
-    check_body<LAT, SLOTS, MAXDEG>(d, t, tile.chunk,
+    Launch L;
+    if (!begin_substep(d, k, L)) return;
+    check_body<LAT, SLOTS, MAXDEG>(d, L.t, L.draw, tile.chunk,
                                    global_ptr(reinterpret_cast<uint64_t*>(lists)) + size_t(blockIdx.x) * (SLOTS * blockDim.x),
                                    global_ptr(listCounts) + blockIdx.x, sForbidden, &sCount);
 }
@@ -379,12 +408,11 @@ __global__ __launch_bounds__(256) void k_perform_batch(const SysDesc* __restrict
     const Tile tile = tiles[blockIdx.x];
     if (tile.replica == kNoReplica) return;
     const SysDesc& d = descs[tile.replica];
-    const gptr<DevState> st = global_ptr(d.state);
-    const uint64_t t = st->substep + k;
-    if (t >= st->stopAt) return;
-    perform_body<LAT>(d, t, tile.chunk, tile.chunks,
+    Launch L;
+    if (!begin_substep(d, k, L)) return;
+    perform_body<LAT>(d, L, tile.chunk, tile.chunks,
                       global_ptr(reinterpret_cast<const uint64_t*>(lists)) + size_t(blockIdx.x) * listStride,
-                      global_ptr(listCounts)[blockIdx.x], global_ptr(st->acceptedByBlock) + tile.chunk, sSum);
+                      global_ptr(listCounts)[blockIdx.x], global_ptr(L.acceptedByBlock) + tile.chunk, sSum);
 }
 
 __global__ void k_advance(DevState* st, uint32_t count) {
