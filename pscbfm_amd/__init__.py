@@ -346,7 +346,7 @@ class UpdaterGPUScBFM_AB_Type:
         self._call("pscbfm_set_launch_shape", int(block), int(slots))
 
     def setPerformBlock(self, block: int = 0) -> None:
-        """Pass-2 threads per workgroup (0: the pass-1 block)."""
+        """Pass-2 threads per workgroup (0: the pass-1 block, at most 128)."""
         self._call("pscbfm_set_perform_block", int(block))
 
     def setGraphChunk(self, substeps: int) -> None:

@@ -77,7 +77,7 @@ public:
     // Threads per workgroup (0 = 256) and monomers per pass-1 thread (0 = 2).
     void setLaunchShape(uint32_t block, uint32_t slots);
     void setGraphChunk(uint32_t substeps) { graphChunk_ = substeps; }
-    // Pass-2 threads per workgroup (0 = the pass-1 block).
+    // Pass-2 threads per workgroup (0 = the pass-1 block, at most 128).
     void setPerformBlock(uint32_t block);
     // R250 mode: from the next run on, every substep draws its species
     // (r250 % nColors) and kernel seed (r250) from this generator in the
