@@ -300,6 +300,7 @@ def main() -> int:
     ap.add_argument("--block", type=int, default=0, help="threads per workgroup (0: default)")
     ap.add_argument("--slots", type=int, default=0, help="monomers per pass-1 thread (0: default)")
     ap.add_argument("--perform-block", type=int, default=0, help="pass-2 threads per workgroup (0: default)")
+    ap.add_argument("--graph-chunk", type=int, default=0, help="substeps per captured graph (0: default)")
     ap.add_argument("--profile-mcs", type=int, default=20)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -338,6 +339,8 @@ def main() -> int:
     u = make_updater(box, xyz, bonds, args.seed, kind, args.block, args.slots)
     if args.perform_block:
         u.setPerformBlock(args.perform_block)
+    if args.graph_chunk:
+        u.setGraphChunk(args.graph_chunk)
     colors = u.colors(xyz.shape[0])
     log(f"initialize done in {time.perf_counter() - tic:.1f} s; species sizes {u.colorSizes().tolist()}; "
         f"device bytes {u.deviceBytes() / 2**20:.0f} MiB")

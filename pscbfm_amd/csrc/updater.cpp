@@ -221,10 +221,12 @@ void Updater::applyLaunchShape() {
     shape_.performBlock = performOverride_;
     shape_.grid = grid_for(layout_.maxColorCount, shape_.block * shape_.slots);
     performGridMax_ = std::max(performGridMax_, shape_.grid);
-    if (graphExec_) {
-        (void)hipGraphExecDestroy(graphExec_);
-        graphExec_ = nullptr;
-    }
+    dropGraphs();
+}
+
+void Updater::dropGraphs() {
+    for (auto& kv : graphs_) (void)hipGraphExecDestroy(kv.second);
+    graphs_.clear();
 }
 
 // UpdaterGPUScBFM_AB_Type.cu:882-986: excluded volume (8 cells per monomer,
@@ -276,10 +278,7 @@ void Updater::checkSystem() const {
 }
 
 void Updater::freeDevice() {
-    if (graphExec_) {
-        (void)hipGraphExecDestroy(graphExec_);
-        graphExec_ = nullptr;
-    }
+    dropGraphs();
     dev_.release();
     if (drawTable_) (void)hipFree(drawTable_);
     drawTable_ = nullptr;
@@ -370,15 +369,15 @@ void Updater::initialize() {
     initialized_ = true;
 }
 
-void Updater::captureGraph() {
-    if (graphExec_) {
-        (void)hipGraphExecDestroy(graphExec_);
-        graphExec_ = nullptr;
-    }
-    const uint32_t nc = nColors();
-    uint32_t g = graphChunk_;
-    if (g == 0) g = nc * std::max<uint32_t>(1u, 32u / std::max<uint32_t>(nc, 1u));
-    graphSubsteps_ = g;
+uint32_t Updater::graphChunk() const {
+    if (graphChunk_) return graphChunk_;
+    const uint32_t nc = std::max<uint32_t>(nColors(), 1u);
+    return nc * std::max<uint32_t>(1u, 256u / nc);
+}
+
+hipGraphExec_t Updater::graphFor(uint32_t g) {
+    const auto it = graphs_.find(g);
+    if (it != graphs_.end()) return it->second;
     hipGraph_t graph = nullptr;
     hip_check(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
     hipError_t err = hipSuccess;
@@ -390,9 +389,12 @@ void Updater::captureGraph() {
     const hipError_t endErr = hipStreamEndCapture(stream_, &graph);
     if (err != hipSuccess) hip_fail(err, "kernel launch during capture");
     hip_check(endErr, "hipStreamEndCapture");
-    const hipError_t instErr = hipGraphInstantiate(&graphExec_, graph, nullptr, nullptr, 0);
+    hipGraphExec_t exec = nullptr;
+    const hipError_t instErr = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
     (void)hipGraphDestroy(graph);
     hip_check(instErr, "hipGraphInstantiate");
+    graphs_[g] = exec;
+    return exec;
 }
 
 void Updater::run(int32_t nMCS, uint32_t* log, bool profile, bool useGraph) {
@@ -500,16 +502,24 @@ void Updater::runChunk(uint64_t S, uint32_t* log, bool profile, bool useGraph) {
             hip_check(launch_advance(dev_.desc.state, 1, stream_), "launch advance");
         }
     } else {
-        uint64_t done = 0;
+        uint64_t left = S;
         if (useGraph) {
-            if (!graphExec_) captureGraph();
-            while (S - done >= graphSubsteps_) {
-                hip_check(hipGraphLaunch(graphExec_, stream_), "hipGraphLaunch");
-                done += graphSubsteps_;
+            // the bulk graph, then the binary digits of the remainder (each
+            // below the bulk size), so every substep runs from a graph
+            const uint32_t G = graphChunk();
+            while (left >= G) {
+                hip_check(hipGraphLaunch(graphFor(G), stream_), "hipGraphLaunch");
+                left -= G;
             }
+            uint32_t g = 1;
+            while (g * 2u <= G) g *= 2u;
+            for (; g >= 1u && left > 0; g >>= 1)
+                if (left >= g) {
+                    hip_check(hipGraphLaunch(graphFor(g), stream_), "hipGraphLaunch");
+                    left -= g;
+                }
         }
-        // Eager tail in blocks of at most 2^20 substeps per advance.
-        uint64_t left = S - done;
+        // Eager launches (graphs off) in blocks of at most 2^20 substeps per advance.
         while (left > 0) {
             const uint32_t blk = uint32_t(std::min<uint64_t>(left, 1u << 20));
             for (uint32_t k = 0; k < blk; ++k) {

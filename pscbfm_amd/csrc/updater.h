@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -76,6 +77,8 @@ public:
     void setColors(const uint8_t* colors, uint32_t n);
     // Threads per workgroup (0 = 256) and monomers per pass-1 thread (0 = 2).
     void setLaunchShape(uint32_t block, uint32_t slots);
+    // Substeps of the bulk graph (0 = nColors * max(1, 256 / nColors)); a
+    // run replays it, then power-of-two graphs cover the remainder.
     void setGraphChunk(uint32_t substeps) { graphChunk_ = substeps; }
     // Pass-2 threads per workgroup (0 = the pass-1 block, at most 128).
     void setPerformBlock(uint32_t block);
@@ -128,7 +131,11 @@ private:
     void requireIndex(uint32_t i, const char* who) const;
     void freeDevice();
     void runChunk(uint64_t S, uint32_t* log, bool profile, bool useGraph);
-    void captureGraph();
+    // Captured graph of `substeps` substeps plus their counter advance,
+    // instantiated on first use.
+    hipGraphExec_t graphFor(uint32_t substeps);
+    void dropGraphs();
+    uint32_t graphChunk() const;
     void syncFromDevice();
     void applyLaunchShape();
 
@@ -160,8 +167,7 @@ private:
     hipStream_t stream_ = nullptr;
     DeviceSystem dev_;
     LaunchShape shape_;
-    hipGraphExec_t graphExec_ = nullptr;
-    uint32_t graphSubsteps_ = 0;
+    std::map<uint32_t, hipGraphExec_t> graphs_;  // by substeps per replay
     uint32_t performGridMax_ = 0;  // largest pass-2 grid used (entries of acceptedByBlock)
     uint64_t substep_ = 0;
     uint64_t acceptedTotal_ = 0;
