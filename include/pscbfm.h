@@ -123,8 +123,9 @@ int pscbfm_set_colors(pscbfm_updater* h, const uint8_t* colors, uint32_t n);
 /* Threads per workgroup (64, 128, 192 or 256; 0 = 256) and monomers per
  * pass-1 thread (1, 2 or 4; 0 = 2).  Results do not depend on the shape. */
 int pscbfm_set_launch_shape(pscbfm_updater* h, uint32_t block, uint32_t slots);
-/* Pass-2 threads per workgroup (64..256, multiple of 64; 0 = the pass-1
- * block, at most 128).  Results do not depend on it. */
+/* Pass-2 threads per workgroup (64..256, multiple of 64; 0 = automatic:
+ * 128 when at least a quarter of the cells are occupied, else the pass-1
+ * block).  Results do not depend on it. */
 int pscbfm_set_perform_block(pscbfm_updater* h, uint32_t block);
 /* Substeps per captured graph (0 = automatic). */
 int pscbfm_set_graph_chunk(pscbfm_updater* h, uint32_t substeps);
@@ -262,7 +263,7 @@ int pscbfm_batch_footprint(pscbfm_batch* b, const pscbfm_updater* staged, uint64
  * when the tile or replica capacity is exhausted. */
 int pscbfm_batch_add(pscbfm_batch* b, const pscbfm_updater* staged, uint32_t* id);
 int pscbfm_batch_remove(pscbfm_batch* b, uint32_t id);
-/* Pass-2 threads per workgroup of the batch (0 = the pass-1 block, at most 128). */
+/* Pass-2 threads per workgroup of the batch (0 = the pass-1 block). */
 int pscbfm_batch_set_perform_block(pscbfm_batch* b, uint32_t block);
 /* The replica stops after n_mcs more sweeps (counted from the last synchronize). */
 int pscbfm_batch_set_target(pscbfm_batch* b, uint32_t id, uint64_t n_mcs);

@@ -346,7 +346,8 @@ class UpdaterGPUScBFM_AB_Type:
         self._call("pscbfm_set_launch_shape", int(block), int(slots))
 
     def setPerformBlock(self, block: int = 0) -> None:
-        """Pass-2 threads per workgroup (0: the pass-1 block, at most 128)."""
+        """Pass-2 threads per workgroup (0: automatic, 128 for systems at least a
+        quarter occupied, else the pass-1 block)."""
         self._call("pscbfm_set_perform_block", int(block))
 
     def setGraphChunk(self, substeps: int) -> None:

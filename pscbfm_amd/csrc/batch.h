@@ -67,7 +67,7 @@ public:
     uint32_t replicaCapacity() const { return replicaCap_; }
     uint64_t deviceBytes() const;
     const LaunchShape& shape() const { return shape_; }
-    // Pass-2 threads per workgroup (0 = the pass-1 block, at most 128).
+    // Pass-2 threads per workgroup (0 = the pass-1 block).
     void setPerformBlock(uint32_t block);
     uint64_t launchedSubsteps() const { return launched_; }
 

@@ -14,11 +14,8 @@ struct LaunchShape {
     uint32_t block = 256;        // pass-1 threads per workgroup (64..256, multiple of 64)
     uint32_t slots = 2;          // monomers per pass-1 thread (1, 2 or 4)
     uint32_t grid = 8;           // workgroups (multiple of 8)
-    uint32_t performBlock = 0;   // pass-2 threads per workgroup (0: automatic)
-    // A pass-2 workgroup walks one compacted list of about a fifth of the
-    // pass-1 capacity, so 128 threads by default: on the dense 512^3 melt
-    // they run the sweep 1.9% faster than 256 and 3% faster than 64.
-    uint32_t passTwoBlock() const { return performBlock ? performBlock : (block > 128u ? 128u : block); }
+    uint32_t performBlock = 0;   // pass-2 threads per workgroup (0: the pass-1 block)
+    uint32_t passTwoBlock() const { return performBlock ? performBlock : block; }
     uint32_t listStride() const { return slots * block; }  // list entries per workgroup
 };
 

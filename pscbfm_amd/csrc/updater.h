@@ -80,7 +80,8 @@ public:
     // Substeps of the bulk graph (0 = nColors * max(1, 256 / nColors)); a
     // run replays it, then power-of-two graphs cover the remainder.
     void setGraphChunk(uint32_t substeps) { graphChunk_ = substeps; }
-    // Pass-2 threads per workgroup (0 = the pass-1 block, at most 128).
+    // Pass-2 threads per workgroup (0 = automatic: 128 for systems at least a
+    // quarter occupied, else the pass-1 block).
     void setPerformBlock(uint32_t block);
     // R250 mode: from the next run on, every substep draws its species
     // (r250 % nColors) and kernel seed (r250) from this generator in the

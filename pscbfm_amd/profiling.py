@@ -29,14 +29,28 @@ KERNELS = {"k_check_batch": "pass 1, batch (k_check_batch)", "k_perform_batch": 
 
 
 _NAME = re.compile(r"\b(k_[a-z0-9_]+?)(?=[<(]|$)")
+_LATTICE = re.compile(r"\b(k_(?:check|perform)(?:_batch)?)<(\d+)")
+SWEEP = ("k_check", "k_perform", "k_check_batch", "k_perform_batch")
 
 
 def _short(name: str) -> str:
-    """Kernel symbol without namespace, template arguments or parameters."""
+    """Kernel symbol without namespace, template arguments or parameters.
+    The sweep kernels keep their lattice instance: `k_check` is the bit
+    lattice, `k_check.u8` the byte lattice (first template argument 1 or 0),
+    so one run that times both never mixes them."""
+    m = _LATTICE.search(name)
+    if m:
+        return m.group(1) + ("" if m.group(2) == "1" else ".u8")
     m = _NAME.search(name)
     if m and m.group(1) in KERNELS:
         return m.group(1)
     return name.split("(")[0].split("::")[-1][-40:] or name[:40]
+
+
+def label(key: str) -> str:
+    base, _, kind = key.partition(".")
+    text = KERNELS.get(base, key)
+    return text + ", uint8 lattice" if kind == "u8" else text
 
 
 def _find(d: Path, suffix: str) -> Path:
@@ -132,30 +146,30 @@ def main(argv: Optional[list] = None) -> int:
             a["max"] = max(a["max"], float(r["MaxNs"]))
             a["share"] += float(r["Percentage"])
         for k, a in sorted(agg.items(), key=lambda kv: -kv[1]["share"]):
-            lines.append(f"| {KERNELS.get(k, k)} | {a['calls']} | {a['total'] / a['calls'] / 1e3:.2f} | "
+            lines.append(f"| {label(k)} | {a['calls']} | {a['total'] / a['calls'] / 1e3:.2f} | "
                          f"{a['min'] / 1e3:.2f} | {a['max'] / 1e3:.2f} | {a['share']:.1f} |")
         lines.append("")
     lines += ["Kernel trace, steady state (mean over the last dispatches of each sweep kernel):", "",
               "| kernel | calls | mean duration (us) | over last | total (us) |", "|---|---|---|---|---|"]
     for k, v in sorted(dur.items(), key=lambda kv: -kv[1]["total_us"]):
-        lines.append(f"| {KERNELS.get(k, k)} | {v['calls']} | {v['mean_us']:.2f} | {v['mean_over']} | "
+        lines.append(f"| {label(k)} | {v['calls']} | {v['mean_us']:.2f} | {v['mean_over']} | "
                      f"{v['total_us']:.1f} |")
     if tr:
         lines += ["", f"HBM traffic per launch (copy reference {tr['copy_bytes']} B: FETCH_SIZE scale "
                       f"{tr['fetch_scale']:.3f}, WRITE_SIZE scale {tr['write_scale']:.3f}):", "",
                   "| kernel | FETCH_SIZE KiB | WRITE_SIZE KiB | read MB | write MB | total MB |", "|---|---|---|---|---|---|"]
         for k, v in tr["kernels"].items():
-            lines.append(f"| {KERNELS.get(k, k)} | {v['fetch_kib']:.0f} | {v['write_kib']:.0f} | "
+            lines.append(f"| {label(k)} | {v['fetch_kib']:.0f} | {v['write_kib']:.0f} | "
                          f"{v['read_bytes'] / 1e6:.1f} | {v['write_bytes'] / 1e6:.1f} | {v['traffic_bytes'] / 1e6:.1f} |")
     counters: Dict[str, Dict[str, float]] = defaultdict(dict)
     for d in args.counters:
         for k, cs in all_counters(d, args.last).items():
             counters[k].update(cs)
-    sweep = [k for k in ("k_check", "k_perform", "k_check_batch", "k_perform_batch") if k in counters]
+    sweep = sorted((k for k in counters if k.partition(".")[0] in SWEEP), key=lambda k: (SWEEP.index(k.partition(".")[0]), k))
     if sweep:
         names = sorted({c for k in sweep for c in counters[k]})
         lines += ["", "Counters per dispatch (mean over the same dispatches):", "",
-                  "| counter | " + " | ".join(KERNELS[k] for k in sweep) + " |",
+                  "| counter | " + " | ".join(label(k) for k in sweep) + " |",
                   "|---|" + "---|" * len(sweep)]
         for c in names:
             lines.append(f"| {c} | " + " | ".join(f"{counters[k].get(c, float('nan')):.4g}" for k in sweep) + " |")
@@ -172,7 +186,7 @@ def main(argv: Optional[list] = None) -> int:
             derived.append(row)
         keys = sorted({c for r in derived for c in r})
         if keys:
-            lines += ["", "| derived | " + " | ".join(KERNELS[k] for k in sweep) + " |", "|---|" + "---|" * len(sweep)]
+            lines += ["", "| derived | " + " | ".join(label(k) for k in sweep) + " |", "|---|" + "---|" * len(sweep)]
             for c in keys:
                 lines.append(f"| {c} | " + " | ".join(f"{r.get(c, float('nan')):.3f}" for r in derived) + " |")
     text = "\n".join(lines) + "\n"
