@@ -322,7 +322,8 @@ def main() -> int:
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py runs on exactly one GPU: the sweep of one system is not sharded across GPUs "
-              "in this build (see DESIGN.md for how it would be)", file=sys.stderr)
+              "(a z-slab decomposition with halo exchange would be the way; it is not implemented)",
+              file=sys.stderr)
         return 2
 
     L = args.box
