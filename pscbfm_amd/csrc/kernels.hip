@@ -211,7 +211,7 @@ __device__ __forceinline__ void check_body(const SysDesc& d, uint64_t t, Substep
     // partner in a row) load from an in-bounds stand-in address and discard
     // the value, so every load of a thread sits in one basic block and all of
     // them are in flight together.
-    // Round 1: own codes and partner ids.
+    // First batch of loads: own codes and partner ids.
     uint32_t j[SLOTS], m[SLOTS], partner[SLOTS][MAXDEG];
     bool live[SLOTS];
 #pragma unroll
@@ -227,7 +227,7 @@ __device__ __forceinline__ void check_body(const SysDesc& d, uint64_t t, Substep
             partner[s][r] = (live[s] && row) ? id : kNoNeighbor;
         }
     }
-    // Round 2: partner codes and the front plane.  CRITICAL.cu:322-323: the
+    // Second batch: partner codes and the front plane.  CRITICAL.cu:322-323: the
     // draw is keyed by the species-local index.
     uint32_t dir[SLOTS], to[SLOTS], partnerCode[SLOTS][MAXDEG], occ[SLOTS];
 #pragma unroll
