@@ -472,6 +472,9 @@ def main() -> int:
             res = serving.run_stream(jobs, log=log, seed=args.seed)
             log(f"  {res['jobs_per_s']:.1f} jobs/s, latency p50 {res['latency_p50_s']:.2f} s "
                 f"p99 {res['latency_p99_s']:.2f} s, GPU busy {res['gpu_busy_fraction']:.2f}; {res['parity']}")
+            if res["melts"]:
+                log(f"  melt jobs ({res['melt_mcs']} MCS) done after {[round(x, 2) for x in res['melt_latency_s']]} s; "
+                    f"{res['melt_parity']}")
             plain = out.get("batched", {}).get("plain_value")
             if plain:
                 res["speedup_vs_plain_loop"] = res["monomer_sweeps_per_s"] / plain

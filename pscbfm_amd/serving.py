@@ -1,8 +1,9 @@
 """Served stream of equilibration jobs over the request loop (Server).
 
-Jobs are drawn like the replica mix (workloads.replica_specs), plus,
-with a small probability, a dense 512^3 melt (32768 chains of 256 at occupied
-fraction 0.5, ordered packing).  Each job asks for a number of sweeps drawn
+Jobs are drawn like the replica mix (workloads.replica_specs).  `melts` of
+them, at arrivals picked at random, are instead a dense 512^3 melt (32768
+chains of 256 at occupied fraction 0.5, ordered packing) asking for
+`melt_mcs` sweeps.  Every other job asks for a number of sweeps drawn
 log-uniformly from [mcs_lo, mcs_hi].  Arrivals are a Poisson process of the
 given rate; a submitter thread hands each job to the server at its arrival
 time (the staging of the job's updater is done beforehand, so the arrival
@@ -37,7 +38,7 @@ class Job:
 
 
 def draw_jobs(rate: float, duration: float, seed: int, mcs_lo: int = 1000, mcs_hi: int = 100000,
-              melt_prob: float = 0.002, max_melts: int = 2) -> List[Job]:
+              melts: int = 1, melt_mcs: int = 1000) -> List[Job]:
     rng = np.random.default_rng(seed)
     times = []
     t = 0.0
@@ -48,18 +49,18 @@ def draw_jobs(rate: float, duration: float, seed: int, mcs_lo: int = 1000, mcs_h
         times.append(t)
     specs = W.replica_specs(len(times), seed + 7)
     systems = W.build_replicas(specs)
+    melt_at = set(rng.choice(len(times), size=min(melts, len(times)), replace=False).tolist()) if times else set()
     jobs = []
-    melts = 0
     for k, (at, sd) in enumerate(zip(times, systems)):
         mcs = int(round(math.exp(rng.uniform(math.log(mcs_lo), math.log(mcs_hi)))))
         kind = "replica"
-        if melts < max_melts and rng.random() < melt_prob:
+        if k in melt_at:
             from . import chain_bonds, gen_linear_chains_ordered
             xyz = gen_linear_chains_ordered(MELT_BOX, MELT_CHAINS, MELT_LEN)
             sd = W.SystemData(MELT_BOX, xyz, chain_bonds(MELT_CHAINS, MELT_LEN), int(rng.integers(1, 2**62)),
                               "ordered")
             kind = "melt"
-            melts += 1
+            mcs = melt_mcs
         jobs.append(Job(sd, mcs, at, kind))
     return jobs
 
@@ -98,12 +99,16 @@ def run_stream(jobs: List[Job], hbm_budget: int = 288 * 10**9, tile_capacity: in
     rng = np.random.default_rng(seed)
     small = [k for k, j in enumerate(jobs) if j.kind == "replica" and j.mcs <= 5000]
     pick = sorted(rng.choice(small, size=min(verify, len(small)), replace=False).tolist()) if small else []
-    same = 0
-    for k in pick:
+    def alone_equals_served(k: int) -> bool:
         u = W.stage(jobs[k].system, lattice_kind, initialize=True)
         u.runSimulationOnGPU(jobs[k].mcs)
-        same += int(np.array_equal(u.getPositions(jobs[k].system.n), srv.result(ids[k], jobs[k].system.n)))
+        ok = np.array_equal(u.getPositions(jobs[k].system.n), srv.result(ids[k], jobs[k].system.n))
         u.cleanup()
+        return bool(ok)
+
+    same = sum(alone_equals_served(k) for k in pick)
+    melt_ids = [k for k, j in enumerate(jobs) if j.kind == "melt"]
+    melt_same = sum(alone_equals_served(k) for k in melt_ids)
     srv.close()
     return {
         "jobs": len(jobs),
@@ -118,4 +123,7 @@ def run_stream(jobs: List[Job], hbm_budget: int = 288 * 10**9, tile_capacity: in
         "launches": st.launches,
         "substeps": st.substeps,
         "parity": f"{same}/{len(pick)} sampled jobs bit-identical to their runs alone",
+        "melt_latency_s": [float(lat[k]) for k in melt_ids],
+        "melt_mcs": [jobs[k].mcs for k in melt_ids],
+        "melt_parity": f"{melt_same}/{len(melt_ids)} melts bit-identical to their runs alone",
     }
