@@ -111,6 +111,10 @@ int pscbfm_set_r250_state(pscbfm_updater* h, const uint32_t* words);
 int pscbfm_r250_from_seed_file(pscbfm_updater* h, const char* path, uint32_t discard);
 int pscbfm_r250_next(pscbfm_updater* h, uint32_t* out, uint64_t n);
 int pscbfm_r250_get_state(const pscbfm_updater* h, uint32_t* words);
+/* R250 mode: substeps of draws kept on the device at once (0 = 2^22, 32 MiB,
+ * also the maximum).  A longer run goes through in chunks of whole sweeps,
+ * with the same result. */
+int pscbfm_set_draw_chunk(pscbfm_updater* h, uint64_t substeps);
 /* Occupancy lattice storage (pscbfm_lattice_kind).  Before initialize.
  * Replaces the USE_BIT_PACKING_LATTICE switch (UpdaterGPUScBFM_AB_Type.cu:13). */
 int pscbfm_set_lattice_kind(pscbfm_updater* h, int kind);

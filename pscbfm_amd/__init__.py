@@ -124,6 +124,7 @@ _SIGNATURES = {
     "pscbfm_r250_from_seed_file": (C.c_int, [_H, C.c_char_p, C.c_uint32]),
     "pscbfm_r250_next": (C.c_int, [_H, _u32p, C.c_uint64]),
     "pscbfm_r250_get_state": (C.c_int, [_H, _u32p]),
+    "pscbfm_set_draw_chunk": (C.c_int, [_H, C.c_uint64]),
     "pscbfm_set_lattice_kind": (C.c_int, [_H, C.c_int]),
     "pscbfm_set_uniform_coloring": (C.c_int, [_H, C.c_int]),
     "pscbfm_set_check_on_initialize": (C.c_int, [_H, C.c_int]),
@@ -323,6 +324,10 @@ class UpdaterGPUScBFM_AB_Type:
         out = np.empty(max(int(n), 1), dtype=np.uint32)
         self._call("pscbfm_r250_next", _ptr(out, C.c_uint32), int(n))
         return out[:int(n)]
+
+    def setDrawChunk(self, substeps: int) -> None:
+        """R250 mode: substeps of draws kept on the device at once (0: 2^22)."""
+        self._call("pscbfm_set_draw_chunk", int(substeps))
 
     def r250State(self) -> np.ndarray:
         out = np.empty(250, dtype=np.uint32)

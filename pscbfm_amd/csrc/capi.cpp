@@ -241,6 +241,10 @@ int pscbfm_r250_next(pscbfm_updater* h, uint32_t* out, uint64_t n) {
     return guarded(h, [&](pscbfm::Updater& u) { u.r250Draw(out, n); });
 }
 
+int pscbfm_set_draw_chunk(pscbfm_updater* h, uint64_t substeps) {
+    return guarded(h, [&](pscbfm::Updater& u) { u.setDrawChunk(substeps); });
+}
+
 int pscbfm_r250_get_state(const pscbfm_updater* h, uint32_t* words) {
     return guarded_const(h, [&](const pscbfm::Updater& u) {
         if (!words) throw std::invalid_argument("[r250State] null output");

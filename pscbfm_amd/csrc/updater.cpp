@@ -418,7 +418,8 @@ void Updater::run(int32_t nMCS, uint32_t* log, bool profile, bool useGraph) {
     const uint64_t S = uint64_t(nMCS) * nc;
     // R250 mode keeps the draw table of one chunk of whole sweeps on the
     // device at a time (at most kMaxDrawSubsteps substeps, 8 bytes each).
-    const uint64_t chunk = r250Mode_ ? std::max<uint64_t>(1u, kMaxDrawSubsteps / nc) * nc : S;
+    const uint64_t drawCap = drawChunk_ ? std::min(drawChunk_, kMaxDrawSubsteps) : kMaxDrawSubsteps;
+    const uint64_t chunk = r250Mode_ ? std::max<uint64_t>(1u, drawCap / nc) * nc : S;
     RunStats total{};
     for (uint64_t done = 0; done < S; done += chunk) {
         const uint64_t part = std::min(chunk, S - done);

@@ -91,6 +91,9 @@ public:
     void r250Draw(uint32_t* out, uint64_t n);
     void r250State(uint32_t* words250) const;
     bool r250Mode() const { return r250Mode_; }
+    // Substeps of explicit draws kept on the device at once (0 = 2^22, 32 MiB);
+    // a longer run goes through in chunks of whole sweeps.
+    void setDrawChunk(uint64_t substeps) { drawChunk_ = substeps; }
     void setMonomerCoordinatesBulk(const int32_t* xyz, uint32_t n);
     void setConnectivityBulk(const uint32_t* pairs, uint64_t nPairs);
     void getPositionsBulk(int32_t* xyz, uint32_t n) const;
@@ -175,6 +178,7 @@ private:
     uint64_t cells_ = 0;
     uint32_t* drawTable_ = nullptr;  // R250 mode: 2 words per substep of one run chunk
     uint64_t drawCapacity_ = 0;      // substeps drawTable_ holds
+    uint64_t drawChunk_ = 0;         // 0: kMaxDrawSubsteps
     RunStats lastRun_;
 };
 

@@ -107,3 +107,28 @@ def test_batch_and_server_reject_r250_updaters():
     with pytest.raises(Exception, match="R250"):
         server.submit(u, 10)
     server.close()
+
+
+@pytest.mark.parametrize("chunk", [2, 5, 14])
+def test_r250_draw_chunks_equal_one_table(chunk):
+    # a run longer than the device draw table goes through in chunks of whole
+    # sweeps; positions, per-substep log and statistics are those of one table
+    s = chains((32, 32, 32), 20, 16, seed=7)
+    gx, glog, stats, colors, u = run_r250(s, P.LATTICE_BIT, [(23, True, False)])
+    u.cleanup()
+    v = make_updater(s, 42)
+    v.loadR250SeedFile(SEEDS, 2)
+    v.setDrawChunk(chunk)
+    st, log = v.run(23, log=True)
+    assert np.array_equal(log, glog)
+    assert np.array_equal(v.getPositions(s.n), gx)
+    assert (st.attempted, st.accepted, st.substeps) == (stats[0].attempted, stats[0].accepted, stats[0].substeps)
+    assert v.r250Next(3).tolist() == u_next_after(23 * (int(colors.max()) + 1))
+    v.cleanup()
+
+
+def u_next_after(n_substeps: int):
+    r = oracle.R250.from_seed_file(SEEDS)
+    for _ in range(2 + 2 * n_substeps):
+        r.rand32()
+    return [r.rand32() for _ in range(3)]
