@@ -7,18 +7,26 @@ import numpy as np
 import pytest
 
 import pscbfm_amd as P
-from helpers import chains, make_updater, ordered_chains, stage_updater, stars
+from helpers import chains, make_updater, oracle_for, ordered_chains, stage_updater, stars
 
 pytestmark = pytest.mark.gpu
 
 
 def _alone(s, seed, mcs):
     u = make_updater(s, seed)
+    colors = u.colors(s.n)
     u.run(mcs)
     x = u.getPositions(s.n)
     acc = u.counters()[1]
     u.cleanup()
-    return x, acc
+    return x, acc, colors
+
+
+def _cpu(s, seed, mcs, colors):
+    """The same system's single-system run on the CPU oracle."""
+    o = oracle_for(s, colors, seed)
+    total, _ = o.run(0, mcs * (int(colors.max()) + 1))
+    return o.xyz, total
 
 
 def _jobs():
@@ -50,9 +58,11 @@ def test_each_served_job_equals_its_run_alone():
     for job, s, seed, mcs in ids:
         info = srv.job(job)
         assert info.state == P.JOB_DONE and info.t_submit <= info.t_admit <= info.t_done
-        x_alone, acc_alone = _alone(s, seed, mcs)
-        assert np.array_equal(srv.result(job, s.n), x_alone)
-        assert info.accepted == acc_alone
+        x_alone, acc_alone, colors = _alone(s, seed, mcs)
+        x_cpu, acc_cpu = _cpu(s, seed, mcs, colors)
+        served = srv.result(job, s.n)
+        assert np.array_equal(served, x_alone) and np.array_equal(served, x_cpu)
+        assert info.accepted == acc_alone == acc_cpu
         srv.release(job)
     srv.close()
 
