@@ -82,6 +82,7 @@ void Server::prepLoop() {
             err = e.what();
         }
         std::lock_guard<std::mutex> lock(mu_);
+        if (!fatal_.empty()) continue;  // the loop has stopped; every pending job is already failed
         if (!err.empty()) {
             finish(p->id, JobState::Failed, err);
             continue;
@@ -136,9 +137,17 @@ void Server::gpuLoop() {
                     continue;
                 }
             }
-            // 2. Admit (device allocation and upload, outside the lock).
+            // 2. Admit (device allocation and upload, outside the lock).  A
+            //    job whose allocation fails fails alone; the loop goes on.
             for (auto& p : admit) {
-                const uint32_t rid = batch_->add(p->layout, p->staged.seed);
+                uint32_t rid = 0;
+                try {
+                    rid = batch_->add(p->layout, p->staged.seed);
+                } catch (const std::exception& e) {
+                    std::lock_guard<std::mutex> lock(mu_);
+                    finish(p->id, JobState::Failed, std::string("admission failed: ") + e.what());
+                    continue;
+                }
                 std::lock_guard<std::mutex> lock(mu_);
                 JobRecord& r = jobs_[p->id];
                 batch_->setTarget(rid, r.nMCS);
@@ -192,6 +201,8 @@ void Server::gpuLoop() {
     } catch (const std::exception& e) {
         std::lock_guard<std::mutex> lock(mu_);
         fatal_ = e.what();
+        toPrepare_.clear();
+        prepared_.clear();
         for (auto& kv : jobs_)
             if (kv.second.state != JobState::Done && kv.second.state != JobState::Failed)
                 finish(kv.first, JobState::Failed, fatal_);
@@ -237,7 +248,7 @@ void Server::release(uint64_t job) {
 
 void Server::drain() {
     std::unique_lock<std::mutex> lock(mu_);
-    cvDone_.wait(lock, [this] { return stats_.done + stats_.failed == stats_.submitted; });
+    cvDone_.wait(lock, [this] { return stats_.done + stats_.failed >= stats_.submitted; });
 }
 
 ServerStats Server::stats() const {
