@@ -115,7 +115,7 @@ int main(int argc, char** argv) {
     Handle u;
     int rc = pscbfm_create(&u.h);
     if (rc) { std::cerr << "create failed: " << pscbfm_create_error() << "\n"; return 1; }
-    if ((rc = pscbfm_set_gpu(u.h, gpu))) return fail(u, "setGpu", rc);
+    // Host-side inputs first, so a bad file is reported before the GPU is touched.
     uint64_t mcs = 0;
     if ((rc = pscbfm_bfm_stage(u.h, infile.c_str(), &mcs))) return fail(u, "reading " + infile, rc);
     if (seedFile.empty()) {
@@ -128,6 +128,7 @@ int main(int argc, char** argv) {
         std::cerr << "RNG rand: " << first[0] << "\nRNG rand: " << first[1] / 4294967296.0 << "\n";
     }
     if ((rc = pscbfm_set_lattice_kind(u.h, lattice))) return fail(u, "setLatticeKind", rc);
+    if ((rc = pscbfm_set_gpu(u.h, gpu))) return fail(u, "setGpu", rc);
     if ((rc = pscbfm_initialize(u.h))) return fail(u, "initialize", rc);
     if ((rc = pscbfm_bfm_write(u.h, outfile.c_str(), mcs, 1))) return fail(u, "write", rc);
     const auto t1 = clock::now();

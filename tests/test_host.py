@@ -425,3 +425,33 @@ def test_library_r250_errors(tmp_path):
         u.loadR250SeedFile(tmp_path / "missing.seeds")
     with pytest.raises(ValueError):
         u.setR250State(np.zeros(10, dtype=np.uint32))
+
+
+def _cli(*args):
+    exe = Path(P.__file__).parent / "pscbfm_run"
+    return subprocess.run([str(exe), *map(str, args)], capture_output=True, text=True, timeout=120)
+
+
+def test_cli_reports_input_errors_before_the_gpu(tmp_path):
+    out = tmp_path / "out.bfm"
+    r = _cli("-i", tmp_path / "missing.bfm", "-m", 1, "-o", out)
+    assert r.returncode == 1 and "cannot open BFM file" in r.stderr
+    r = _cli("-i", GOLDEN / "Ring_N6_n6.bfm", "-m", 1, "-o", out)  # non-periodic fixture
+    assert r.returncode == 1 and "periodic" in r.stderr
+    short = tmp_path / "short.seeds"
+    short.write_bytes(b"\0" * 40)
+    r = _cli("-i", GOLDEN / "LinearChain_l1024_n64.bfm.gz", "-m", 1, "-e", short, "-o", out)
+    assert r.returncode == 1 and "fewer than 257" in r.stderr
+    r = _cli("-i", GOLDEN / "LinearChain_l1024_n64.bfm.gz", "-m", 1, "-e", short, "--seed", 3, "-o", out)
+    assert r.returncode == 1 and "give one of them" in r.stderr
+    r = _cli("-m", 1)
+    assert r.returncode == 1 and "required" in r.stderr
+
+
+def test_cli_seed_file_prints_the_reference_known_answers(tmp_path):
+    # mainCUDASimulatorGPUScBFM_AB_Type.cpp:138-139: the two draws before the run
+    r = _cli("-i", GOLDEN / "LinearChain_l1024_n64.bfm.gz", "-m", 1, "-e", GOLDEN / "resultNormPscBFM.seeds",
+             "-o", tmp_path / "out.bfm")
+    assert "RNG rand: 4223731124" in r.stderr and "RNG rand: 0.803876" in r.stderr
+    # without a GPU the run stops at device selection; with one it completes
+    assert r.returncode == 0 or "setGpu" in r.stderr
