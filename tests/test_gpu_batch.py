@@ -168,3 +168,32 @@ def test_batch_autotune_picks_a_candidate():
     block, slots, trials = P.autotune_batch(staged, mcs=2)
     assert (block, slots) in P.TUNE_CANDIDATES
     assert trials[(block, slots)] == min(trials.values())
+
+
+def test_batch_mixing_degree_buckets_matches_alone_and_oracle():
+    # the batch kernel runs at the widest neighbour table of its replicas (8
+    # here) over replicas whose own tables are narrower (0, 2, 4)
+    from helpers import GOLDEN, Sys, oracle_for
+    from pscbfm_amd.bfm import read_bfm
+    gel = read_bfm(GOLDEN / "Hydrogel_NStar_117_NoPerXYZ64.bfm")
+    systems = [
+        (stars((64, 64, 64), 20, 8, 10, seed=3), 5),
+        (Sys(gel.box, gel.xyz, gel.bonds), 6),
+        (chains((32, 32, 32), 20, 16, seed=7), 42),
+        (stars((64, 64, 64), 40, 4, 29, seed=9), 77),
+        (chains((16, 16, 16), 1, 1, seed=2), 8),
+    ]
+    b = P.Batch(tile_capacity=128, replica_capacity=8)
+    ids = [b.add(stage_updater(s, seed)) for s, seed in systems]
+    b.run(25)
+    for rid, (s, seed) in zip(ids, systems):
+        u = make_updater(s, seed)
+        colors = u.colors(s.n)
+        u.run(25)
+        x_alone = u.getPositions(s.n)
+        u.cleanup()
+        o = oracle_for(s, colors, seed)
+        total, _ = o.run(0, 25 * (int(colors.max()) + 1))
+        got = b.positions(rid, s.n)
+        assert np.array_equal(got, x_alone) and np.array_equal(got, o.xyz)
+        assert b.info(rid).accepted == total
