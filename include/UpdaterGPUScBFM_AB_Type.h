@@ -80,6 +80,7 @@ public:
     /* ---- extensions -------------------------------------------------------- */
     void checkSystem() { check(pscbfm_check_system(h_)); }
     void setSeed(uint64_t seed) { check(pscbfm_set_seed(h_, seed)); }
+    void setStartMcs(uint64_t mcs) { check(pscbfm_set_start_mcs(h_, mcs)); }
     // R250 mode: the 250 state words in draw order, or a LeMonADE seed file
     // (state = words 7..256) followed by `discard` draws.
     void setR250State(const uint32_t* words250) { check(pscbfm_set_r250_state(h_, words250)); }

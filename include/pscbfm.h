@@ -93,6 +93,13 @@ int pscbfm_cleanup(pscbfm_updater* h);
 /* Key of the counter-based substep draw; replaces the seeding of LeMonADE's
  * R250 stream that drives CRITICAL.cu:509-510.  Before initialize. */
 int pscbfm_set_seed(pscbfm_updater* h, uint64_t seed);
+/* Age of the input state in sweeps (a BFM frame's !mcs), before initialize.
+ * The substep counter, and with it the counter draw, starts at
+ * mcs * nColors.  A run resumed from a checkpoint of age m then continues
+ * the uninterrupted trajectory bit for bit; the reference cannot, since its
+ * R250 state is not saved with the snapshot (mainCUDASimulatorGPUScBFM_AB_Type.cpp:155,160).
+ * Batches and the request loop start a staged system's replica there too. */
+int pscbfm_set_start_mcs(pscbfm_updater* h, uint64_t mcs);
 /* R250 mode: the per-substep species and kernel seed come from the R250
  * stream the reference draws them from (CRITICAL.cu:509-510: species =
  * r250 % nSpecies, then seed = r250), instead of the counter draw.  Any time;

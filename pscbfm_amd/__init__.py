@@ -120,6 +120,7 @@ _SIGNATURES = {
     "pscbfm_check_system": (C.c_int, [_H]),
     "pscbfm_cleanup": (C.c_int, [_H]),
     "pscbfm_set_seed": (C.c_int, [_H, C.c_uint64]),
+    "pscbfm_set_start_mcs": (C.c_int, [_H, C.c_uint64]),
     "pscbfm_set_r250_state": (C.c_int, [_H, _u32p]),
     "pscbfm_r250_from_seed_file": (C.c_int, [_H, C.c_char_p, C.c_uint32]),
     "pscbfm_r250_next": (C.c_int, [_H, _u32p, C.c_uint64]),
@@ -306,6 +307,12 @@ class UpdaterGPUScBFM_AB_Type:
     # -- extensions -----------------------------------------------------------
     def setSeed(self, seed: int) -> None:
         self._call("pscbfm_set_seed", int(seed) & 0xFFFFFFFFFFFFFFFF)
+
+    def setStartMcs(self, mcs: int) -> None:
+        """Age of the input state in sweeps (before initialize): the counter
+        draw continues from sweep `mcs`, so a resumed run equals the
+        uninterrupted one."""
+        self._call("pscbfm_set_start_mcs", int(mcs))
 
     def setR250State(self, words: Sequence[int]) -> None:
         """R250 mode: the substep draws come from this R250 state (250 words in

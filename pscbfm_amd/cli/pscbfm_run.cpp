@@ -14,7 +14,9 @@
 // (mainCUDASimulatorGPUScBFM_AB_Type.cpp:138-139), so the stream the sweep
 // sees is the reference's.  Without -e they come from the counter draw keyed
 // by --seed, or by a random key when neither is given (the reference seeds
-// its R250 at random then).
+// its R250 at random then).  The counter draw continues from the input
+// frame's age, so a run resumed from the last frame of its own output file
+// (which the reader takes) equals the uninterrupted run.
 #include <getopt.h>
 
 #include <chrono>
@@ -128,6 +130,9 @@ int main(int argc, char** argv) {
         std::cerr << "RNG rand: " << first[0] << "\nRNG rand: " << first[1] / 4294967296.0 << "\n";
     }
     if ((rc = pscbfm_set_lattice_kind(u.h, lattice))) return fail(u, "setLatticeKind", rc);
+    // the draws continue from the frame's age: resuming from a saved frame
+    // continues the uninterrupted run
+    if ((rc = pscbfm_set_start_mcs(u.h, mcs))) return fail(u, "setStartMcs", rc);
     if ((rc = pscbfm_set_gpu(u.h, gpu))) return fail(u, "setGpu", rc);
     if ((rc = pscbfm_initialize(u.h))) return fail(u, "initialize", rc);
     if ((rc = pscbfm_bfm_write(u.h, outfile.c_str(), mcs, 1))) return fail(u, "write", rc);

@@ -99,7 +99,7 @@ uint64_t Batch::footprint(const SystemLayout& L) const {
     return DeviceSystem::footprint(L, latticeKind_, false, tilesNeeded(L));
 }
 
-uint32_t Batch::add(const SystemLayout& L, uint64_t seed) {
+uint32_t Batch::add(const SystemLayout& L, uint64_t seed, uint64_t startMcs) {
     uint32_t id = 0;
     while (id < replicaCap_ && replicas_[id].live) ++id;
     if (id == replicaCap_) throw std::runtime_error("batch: replica capacity exhausted");
@@ -117,8 +117,10 @@ uint32_t Batch::add(const SystemLayout& L, uint64_t seed) {
     r.seed = seed;
     r.dev.reset(new DeviceSystem());
     r.dev->allocate(L, latticeKind_, seed, /*withLists=*/false, nT, stream_);
+    const uint64_t start = startMcs * uint64_t(L.colors.size());
     DevState st = r.dev->readState(stream_);
-    st.stopAt = 0;  // idle until a target is set
+    st.substep = start;
+    st.stopAt = start;  // idle until a target is set
     r.dev->writeState(st, stream_);
 
     // Workgroup slots: one XCD for a small replica, all XCDs for a large one.
@@ -147,9 +149,11 @@ uint32_t Batch::add(const SystemLayout& L, uint64_t seed) {
     usedTiles_ += nT;
     descs_[id] = r.dev->desc;
     r.live = true;
-    r.pendingStop = 0;
+    r.pendingStop = start;
     r.info = BatchReplicaInfo{};
     r.info.live = true;
+    r.info.substep = start;
+    r.info.stopAt = start;
     r.info.nColors = uint32_t(L.colors.size());
     r.info.nMonomers = L.nMonomers;
     r.info.tiles = nT;

@@ -47,7 +47,9 @@ public:
     uint32_t tilesNeeded(const SystemLayout& layout) const;
     // Adds a replica (substep 0, stopped until a target is set); returns its id.
     // Throws std::runtime_error when tile or replica capacity is exhausted.
-    uint32_t add(const SystemLayout& layout, uint64_t seed);
+    // startMcs: age of the replica's state in sweeps; its substep counter
+    // starts at startMcs * nColors, as an updater's does.
+    uint32_t add(const SystemLayout& layout, uint64_t seed, uint64_t startMcs = 0);
     void remove(uint32_t id);
     // stopAt = (substeps done) + nMCS * nColors for the replica.
     void setTarget(uint32_t id, uint64_t nMCS);

@@ -252,6 +252,10 @@ int pscbfm_r250_get_state(const pscbfm_updater* h, uint32_t* words) {
     });
 }
 
+int pscbfm_set_start_mcs(pscbfm_updater* h, uint64_t mcs) {
+    return guarded(h, [&](pscbfm::Updater& u) { u.setStartMcs(mcs); });
+}
+
 int pscbfm_set_lattice_kind(pscbfm_updater* h, int kind) {
     return guarded(h, [&](pscbfm::Updater& u) {
         if (kind < 0) throw std::invalid_argument("[setLatticeKind] unknown kind");
@@ -618,7 +622,7 @@ int pscbfm_batch_add(pscbfm_batch* b, const pscbfm_updater* staged, uint32_t* id
     if (!staged) return PSCBFM_NULL_HANDLE;
     return batch_guarded(b, [&](pscbfm::Batch& e) {
         require_counter_draws(staged->engine);
-        const uint32_t r = e.add(staged->engine.buildLayout(), staged->engine.seed());
+        const uint32_t r = e.add(staged->engine.buildLayout(), staged->engine.seed(), staged->engine.startMcs());
         if (id) *id = r;
     });
 }

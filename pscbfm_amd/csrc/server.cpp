@@ -142,7 +142,7 @@ void Server::gpuLoop() {
             for (auto& p : admit) {
                 uint32_t rid = 0;
                 try {
-                    rid = batch_->add(p->layout, p->staged.seed);
+                    rid = batch_->add(p->layout, p->staged.seed, p->staged.startMcs);
                 } catch (const std::exception& e) {
                     std::lock_guard<std::mutex> lock(mu_);
                     finish(p->id, JobState::Failed, std::string("admission failed: ") + e.what());

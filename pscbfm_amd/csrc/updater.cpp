@@ -185,6 +185,11 @@ void Updater::setSeed(uint64_t seed) {
     seed_ = seed;
 }
 
+void Updater::setStartMcs(uint64_t mcs) {
+    if (initialized_) throw std::runtime_error("[setStartMcs] the start is fixed at initialize");
+    startMcs_ = mcs;
+}
+
 void Updater::setLatticeKind(uint32_t kind) {
     if (initialized_) throw std::runtime_error("[setLatticeKind] the lattice kind is fixed at initialize");
     if (kind != kLatticeU8 && kind != kLatticeBit) throw std::invalid_argument("[setLatticeKind] unknown kind");
@@ -340,6 +345,7 @@ StagedSystem Updater::staged() const {
     if (hasColorOverride_) s.colors = colorOverride_;
     s.uniformColors = uniformColors_;
     s.seed = seed_;
+    s.startMcs = startMcs_;
     return s;
 }
 
@@ -374,7 +380,12 @@ void Updater::initialize() {
 
     performGridMax_ = 0;
     applyLaunchShape();
-    substep_ = 0;
+    substep_ = startMcs_ * nColors();
+    if (substep_ != 0) {
+        DevState st = dev_.readState(stream_);
+        st.substep = substep_;
+        dev_.writeState(st, stream_);
+    }
     acceptedTotal_ = 0;
     initialized_ = true;
 }

@@ -38,6 +38,7 @@ struct StagedSystem {
     std::vector<uint8_t> colors;  // empty: automatic coloring
     bool uniformColors = true;
     uint64_t seed = 0;
+    uint64_t startMcs = 0;         // age of the input state in sweeps (resume point)
 };
 
 // The 108-bond check of UpdaterGPUScBFM_AB_Type.cu:514-530, then the layout.
@@ -71,6 +72,11 @@ public:
 
     // --- extensions ---------------------------------------------------------
     void setSeed(uint64_t seed);
+    // Age of the input state in sweeps: the substep counter, and with it the
+    // counter draw, starts at startMcs * nColors, so a run resumed from a
+    // checkpoint of age m continues the uninterrupted trajectory.
+    void setStartMcs(uint64_t mcs);
+    uint64_t startMcs() const { return startMcs_; }
     void setLatticeKind(uint32_t kind);
     void setUniformColoring(bool on) { uniformColors_ = on; }
     void setCheckOnInitialize(bool on) { checkOnInit_ = on; }
@@ -157,6 +163,7 @@ private:
     std::vector<uint8_t> colorOverride_;
     bool hasColorOverride_ = false;
     uint64_t seed_ = 0;
+    uint64_t startMcs_ = 0;
     bool r250Mode_ = false;
     R250 r250_;
     uint32_t latticeKind_ = kLatticeBit;

@@ -21,6 +21,7 @@ def _updater_from(path, seed):
     u.setGpu(0)
     mcs = u.loadBfm(path)
     u.setSeed(seed)
+    u.setStartMcs(mcs)  # the simulator continues the draws from the frame's age
     u.initialize()
     return u, mcs
 
@@ -60,7 +61,7 @@ def test_cli_seed_file_and_reference_fixture(tmp_path):
     assert len(got.frames) == 3
     u = P.UpdaterGPUScBFM_AB_Type()
     u.setGpu(0)
-    u.loadBfm(src)
+    u.setStartMcs(u.loadBfm(src))
     u.initialize()
     u.loadR250SeedFile(seeds, 2)
     for k in (1, 2):
