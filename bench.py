@@ -384,7 +384,8 @@ def main() -> int:
     tfile = ROOT / "profiles" / f"dense_melt_{L}_{args.lattice}.json"
     if tfile.exists() and args.chain == 256 and abs(args.phi - 0.5) < 1e-9:
         tr = json.loads(tfile.read_text()).get("traffic") or {}
-        traffic = tr.get("kernels", {}).get("k_check", {}).get("traffic_bytes")
+        key = "k_check" if args.lattice == "bit" else "k_check.u8"  # the profile's name of this lattice's pass 1
+        traffic = tr.get("kernels", {}).get(key, {}).get("traffic_bytes")
 
     out = {
         "metric": METRIC,
