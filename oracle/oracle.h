@@ -14,6 +14,16 @@
  *   oracle_hash            golden vectors evaluated from the constants and
  *                          shifts of CRITICAL.cu:2-11 (tests/golden)
  *   oracle_splitmix64      published SplitMix64 known answers
+ *   oracle_substep_draw    the SplitMix64 pins plus the library's own draw on
+ *                          a grid of (seed, t, nColors)
+ *   oracle_r250_*          the reference's known answers 4223731124 and
+ *                          0.803876 for tests/resultNormPscBFM.seeds
+ *                          (mainCUDASimulatorGPUScBFM_AB_Type.cpp:138-139),
+ *                          and the stream recurrence on 2000 outputs; any
+ *                          adjustment LeMonADE may make to other loaded
+ *                          words is parity unpinned
+ *   oracle_direction       the direction table of UpdaterGPUScBFM_AB_Type.cu:534-545
+ *                          read from the reference text (tests/golden)
  *   oracle_standard_bondset  the 108 vectors of tests/LinearChain_l1024_n64.bfm
  *   oracle_bond_index      brute-force &7 aliasing identities
  *   oracle_zcurve_index    brute-force bit interleave
@@ -21,7 +31,9 @@
  *   oracle_run             the naive O(n^2) definition (oracle_run_naive),
  *                          order independence (shuffled processing order),
  *                          excluded-volume/bond invariants, exact single
- *                          monomer and dimer acceptance
+ *                          monomer and dimer acceptance; explicit draw lists
+ *                          equal to the counter draw give the counter run
+ *   oracle_check_system    planted overlaps and forbidden bonds on tiny systems
  *   oracle_sequential_run  statistics only (acceptance, bond lengths)
  */
 #ifndef PSCBFM_ORACLE_H
