@@ -170,7 +170,8 @@ def test_batch_autotune_picks_a_candidate():
     assert trials[(block, slots)] == min(trials.values())
 
 
-def test_batch_mixing_degree_buckets_matches_alone_and_oracle():
+@pytest.mark.parametrize("kind", [P.LATTICE_BIT, P.LATTICE_U8])
+def test_batch_mixing_degree_buckets_matches_alone_and_oracle(kind):
     # the batch kernel runs at the widest neighbour table of its replicas (8
     # here) over replicas whose own tables are narrower (0, 2, 4)
     from helpers import GOLDEN, Sys, oracle_for
@@ -183,11 +184,11 @@ def test_batch_mixing_degree_buckets_matches_alone_and_oracle():
         (stars((64, 64, 64), 40, 4, 29, seed=9), 77),
         (chains((16, 16, 16), 1, 1, seed=2), 8),
     ]
-    b = P.Batch(tile_capacity=128, replica_capacity=8)
-    ids = [b.add(stage_updater(s, seed)) for s, seed in systems]
+    b = P.Batch(tile_capacity=128, replica_capacity=8, lattice_kind=kind)
+    ids = [b.add(stage_updater(s, seed, kind)) for s, seed in systems]
     b.run(25)
     for rid, (s, seed) in zip(ids, systems):
-        u = make_updater(s, seed)
+        u = make_updater(s, seed, kind)
         colors = u.colors(s.n)
         u.run(25)
         x_alone = u.getPositions(s.n)
