@@ -1,7 +1,8 @@
 """GPU parity on edge shapes, bit for bit against the oracle: the
 reference's own ring and hydrogel fixtures (run in periodic boxes of their
-size), degree-8 networks (the widest neighbour table), the smallest and
-strongly non-cubic boxes, and coordinates unfolded far from the box."""
+size), degree-8 networks (the widest neighbour table), the smallest, the largest
+(2^32 cells) and strongly non-cubic boxes, and coordinates unfolded far from
+the box."""
 import numpy as np
 import pytest
 
@@ -57,3 +58,18 @@ def test_far_unfolded_coordinates_shift_the_result():
     fx, flog, ox, olog = gpu_vs_oracle(far, seed=21, n_mcs=100)
     assert np.array_equal(flog, glog) and np.array_equal(flog, olog)
     assert np.array_equal(fx, gx + shift) and np.array_equal(fx, ox)
+
+
+def test_largest_box_across_the_wrapping_corner():
+    # 2048 x 2048 x 1024 = 2^32 cells, the most a 32-bit cell code spans; the
+    # chains straddle the corner where all three axes wrap, so the top code
+    # bits and the periodic images are in play on every move
+    small = chains((32, 32, 32), 12, 16, seed=4)
+    box = (2048, 2048, 1024)
+    shift = np.array([box[0] - 16, box[1] - 16, box[2] - 16], dtype=np.int32)
+    s = Sys(box, small.xyz + shift, small.bonds)
+    print(f"largest box {box}: {s.n} monomers", flush=True)
+    gx, glog, ox, olog = gpu_vs_oracle(s, seed=13, n_mcs=40)
+    assert np.array_equal(glog, olog)
+    assert np.array_equal(gx, ox)
+    assert glog.sum() > 0
