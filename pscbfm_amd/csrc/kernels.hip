@@ -11,7 +11,7 @@
 //    Z-order bond key of two codes;
 //  * both passes are bound by the number of gather instructions and memory
 //    latency more than by bandwidth: a pass-1 thread handles SLOTS monomers
-//    and issues all their loads in two rounds (own codes and partner ids;
+//    and issues all their loads in two batches (own codes and partner ids;
 //    then partner codes and the front plane together, from one basic block);
 //    the 3x3 front plane is read as at most four 64-bit words;
 //  * pass 1 compacts the surviving proposals of its workgroup into a list
@@ -26,10 +26,12 @@
 //    device-scope atomics are executed beyond the per-XCD L2 and cost a
 //    memory transaction each;
 //  * workgroup w runs on XCD w%8: a single system renumbers workgroups so
-//    each XCD takes one contiguous eighth of the species; a batch keeps all
-//    workgroups of a replica on one XCD through its tile table;
-//  * the species and kernel seed of a substep are drawn on the device from
-//    the substep counter, so launches carry no per-substep host data.
+//    each XCD takes one contiguous eighth of the species; a batch's tile
+//    table keeps a small replica's workgroups on one XCD and spreads a large
+//    one over all eight;
+//  * the species and kernel seed of a substep come from the device state:
+//    the counter draw of (seed, substep), or an explicit draw table (R250
+//    mode), so launches carry no per-substep host data.
 #include "kernels.h"
 
 namespace pscbfm {
