@@ -230,20 +230,19 @@ void Batch::dropGraphs() {
     graphs_.clear();
 }
 
-// Graph of kGraphSubsteps substeps over workgroups [0, grid), then the
-// counter advance.
-hipGraphExec_t Batch::graphFor(uint32_t grid) {
-    auto it = graphs_.find(grid);
+hipGraphExec_t Batch::graphFor(uint32_t grid, uint32_t substeps) {
+    const auto key = std::make_pair(grid, substeps);
+    auto it = graphs_.find(key);
     if (it != graphs_.end()) return it->second;
     BatchLaunch b{dDescs_, dTiles_, dLists_, dListCounts_, grid, graphDegree_, latticeKind_};
     hipGraph_t g = nullptr;
     hip_check(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
     hipError_t err = hipSuccess;
-    for (uint32_t k = 0; k < kGraphSubsteps && err == hipSuccess; ++k) {
+    for (uint32_t k = 0; k < substeps && err == hipSuccess; ++k) {
         err = launch_check_batch(b, k, shape_, stream_);
         if (err == hipSuccess) err = launch_perform_batch(b, k, shape_, stream_);
     }
-    if (err == hipSuccess) err = launch_advance_batch(dDescs_, replicaCap_, kGraphSubsteps, stream_);
+    if (err == hipSuccess) err = launch_advance_batch(dDescs_, replicaCap_, substeps, stream_);
     const hipError_t endErr = hipStreamEndCapture(stream_, &g);
     if (err != hipSuccess) hip_fail(err, "kernel launch during capture");
     hip_check(endErr, "hipStreamEndCapture");
@@ -251,7 +250,7 @@ hipGraphExec_t Batch::graphFor(uint32_t grid) {
     const hipError_t ie = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
     (void)hipGraphDestroy(g);
     hip_check(ie, "hipGraphInstantiate");
-    graphs_[grid] = exec;
+    graphs_[key] = exec;
     return exec;
 }
 
@@ -261,21 +260,16 @@ void Batch::launch(uint64_t substeps) {
     if (liveReplicas() == 0 || substeps == 0) return;
     const uint32_t grid = activeGrid_;
     uint64_t left = substeps;
-    if (left >= kGraphSubsteps) {
-        hipGraphExec_t exec = graphFor(grid);
-        while (left >= kGraphSubsteps) {
-            hip_check(hipGraphLaunch(exec, stream_), "hipGraphLaunch");
-            left -= kGraphSubsteps;
-        }
+    while (left >= kGraphSubsteps) {
+        hip_check(hipGraphLaunch(graphFor(grid, kGraphSubsteps), stream_), "hipGraphLaunch");
+        left -= kGraphSubsteps;
     }
-    if (left) {
-        BatchLaunch b{dDescs_, dTiles_, dLists_, dListCounts_, grid, graphDegree_, latticeKind_};
-        for (uint32_t k = 0; k < left; ++k) {
-            hip_check(launch_check_batch(b, k, shape_, stream_), "launch pass 1");
-            hip_check(launch_perform_batch(b, k, shape_, stream_), "launch pass 2");
+    // the binary digits of the remainder, each below the bulk size
+    for (uint32_t g = kGraphSubsteps / 2; g >= 1u && left > 0; g >>= 1)
+        if (left >= g) {
+            hip_check(hipGraphLaunch(graphFor(grid, g), stream_), "hipGraphLaunch");
+            left -= g;
         }
-        hip_check(launch_advance_batch(dDescs_, replicaCap_, uint32_t(left), stream_), "advance");
-    }
     launched_ += substeps;
 }
 

@@ -9,6 +9,7 @@
 #include <cstdint>
 #include <map>
 #include <memory>
+#include <utility>
 #include <vector>
 
 #include "common.h"
@@ -84,7 +85,9 @@ private:
         uint64_t pendingStop = 0;
     };
     void uploadTables();
-    hipGraphExec_t graphFor(uint32_t grid);
+    // Captured graph of `substeps` substeps over workgroups [0, grid) plus
+    // their counter advance, instantiated on first use.
+    hipGraphExec_t graphFor(uint32_t grid, uint32_t substeps);
     void dropGraphs();
     void growTiles(uint32_t needed);
 
@@ -114,8 +117,10 @@ private:
     // rounded up to a power of two, so a sparsely filled table does not pay
     // for its idle workgroups.  One captured graph per grid size in use.
     uint32_t activeGrid_ = 8;
-    std::map<uint32_t, hipGraphExec_t> graphs_;
-    static constexpr uint32_t kGraphSubsteps = 16;
+    std::map<std::pair<uint32_t, uint32_t>, hipGraphExec_t> graphs_;  // by (grid, substeps)
+    // Substeps of the bulk graph; a launch replays it, then power-of-two
+    // graphs cover the remainder.
+    static constexpr uint32_t kGraphSubsteps = 64;
     uint64_t launched_ = 0;
 };
 
