@@ -302,7 +302,9 @@ def main() -> int:
     ap.add_argument("--perform-block", type=int, default=0, help="pass-2 threads per workgroup (0: default)")
     ap.add_argument("--graph-chunk", type=int, default=0, help="substeps per captured graph (0: default)")
     ap.add_argument("--profile-mcs", type=int, default=20)
-    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--cpu-seconds", type=float, default=20.0,
+                    help="target seconds of the all-cores oracle sample (its calibration run overestimates, "
+                         "so the sample lands at about half)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-single-seconds", type=float, default=5.0,
                     help="seconds for each single-core CPU figure (0 = skip them)")
