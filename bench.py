@@ -209,7 +209,7 @@ def network_section(seed: int, mcs: int, kind: int, c_l: float) -> dict:
 
 
 def batched_vs_plain(n_rep: int, mcs: int, seed: int, kind: int, tune_sample: int, tune_mcs: int,
-                     plain: bool = True) -> dict:
+                     plain: bool = True, shape=(0, 0)) -> dict:
     """Replica mix: one batch (autotuned shape, graph-replayed launches over
     every replica) against the plain loop of one updater per system, each
     called with runSimulationOnGPU(mcs) in turn.  Every replica's final
@@ -223,12 +223,19 @@ def batched_vs_plain(n_rep: int, mcs: int, seed: int, kind: int, tune_sample: in
     log(f"replica mix: {n_rep} systems, {n_total} monomers ({n_ordered} by ordered fallback), "
         f"built and staged in {time.perf_counter() - tic:.1f} s")
 
-    rng = np.random.default_rng(seed + 1)
-    pick = sorted(rng.choice(n_rep, size=min(tune_sample, n_rep), replace=False).tolist())
-    tic = time.perf_counter()
-    block, slots, trials = P.autotune_batch([staged[i] for i in pick], tune_mcs, kind)
-    log(f"autotune on {len(pick)} replicas: block {block} slots {slots} ({time.perf_counter() - tic:.1f} s); "
-        + ", ".join(f"{b}x{s_}:{ms:.3f}" for (b, s_), ms in sorted(trials.items())))
+    if shape[0] and shape[1]:
+        # a fixed shape (profiling runs pin it: the autotuner's timings shift
+        # under a profiler)
+        block, slots = shape
+        trials = {}
+        log(f"batch shape fixed: block {block} slots {slots}")
+    else:
+        rng = np.random.default_rng(seed + 1)
+        pick = sorted(rng.choice(n_rep, size=min(tune_sample, n_rep), replace=False).tolist())
+        tic = time.perf_counter()
+        block, slots, trials = P.autotune_batch([staged[i] for i in pick], tune_mcs, kind)
+        log(f"autotune on {len(pick)} replicas: block {block} slots {slots} ({time.perf_counter() - tic:.1f} s); "
+            + ", ".join(f"{b}x{s_}:{ms:.3f}" for (b, s_), ms in sorted(trials.items())))
 
     batch = P.Batch(0, n_rep, kind, block, slots)
     tic = time.perf_counter()
@@ -311,6 +318,8 @@ def main() -> int:
     ap.add_argument("--replicas", type=int, default=1024, help="systems of the batched comparison")
     ap.add_argument("--replica-mcs", type=int, default=500)
     ap.add_argument("--tune-sample", type=int, default=64)
+    ap.add_argument("--batch-block", type=int, default=0, help="fixed batch block (with --batch-slots; 0: autotune)")
+    ap.add_argument("--batch-slots", type=int, default=0, help="fixed batch slots (with --batch-block; 0: autotune)")
     ap.add_argument("--tune-mcs", type=int, default=5)
     ap.add_argument("--no-plain", action="store_true", help="batched section without the plain per-system loop")
     ap.add_argument("--no-batch", action="store_true")
@@ -464,7 +473,8 @@ def main() -> int:
         out["network"] = network_section(args.seed, args.network_mcs, kind, c_l)
     if not args.no_batch and args.replicas > 0:
         out["batched"] = batched_vs_plain(args.replicas, args.replica_mcs, args.seed, kind, args.tune_sample,
-                                          args.tune_mcs, plain=not args.no_plain)
+                                          args.tune_mcs, plain=not args.no_plain,
+                                          shape=(args.batch_block, args.batch_slots))
     if not args.no_serve and args.serve_rates:
         from pscbfm_amd import serving
         out["serving"] = {}
