@@ -119,17 +119,21 @@ void DeviceSystem::allocate(const SystemLayout& L, uint32_t latticeKind, uint64_
     up(P_NBR, L.nbr.data(), uint64_t(L.nbr.size()) * 4);
     up(P_COLORS, L.colors.data(), uint64_t(L.colors.size()) * sizeof(ColorInfo));
     up(P_BOND, L.bondMask, 64);
+    // Host staging buffers live until the stream synchronizes below: the
+    // copies are asynchronous.
+    std::vector<uint8_t> latBytes;
+    std::vector<uint32_t> latBits;
     if (latticeKind == kLatticeU8) {
-        std::vector<uint8_t> lat(cells, 0);
-        for (uint32_t i = 0; i < L.nMonomers; ++i) lat[L.pos[L.slotOf[i]]] = 1;
-        up(P_LAT, lat.data(), cells);
+        latBytes.assign(cells, 0);
+        for (uint32_t i = 0; i < L.nMonomers; ++i) latBytes[L.pos[L.slotOf[i]]] = 1;
+        up(P_LAT, latBytes.data(), cells);
     } else {
-        std::vector<uint32_t> lat(latWords, 0);
+        latBits.assign(latWords, 0);
         for (uint32_t i = 0; i < L.nMonomers; ++i) {
             const uint32_t c = L.pos[L.slotOf[i]];
-            lat[c >> 5] |= 1u << (c & 31u);
+            latBits[c >> 5] |= 1u << (c & 31u);
         }
-        up(P_LAT, lat.data(), latWords * 4);
+        up(P_LAT, latBits.data(), latWords * 4);
     }
     DevState st{};
     st.stopAt = std::numeric_limits<uint64_t>::max();
