@@ -172,13 +172,19 @@ struct MortonCodec {
 // Device-resident descriptors.
 
 // Per-color ("species") layout, indexed by the species drawn on the device.
+// 16-bit partner codes (layout.h): no partner, and "read the full matrix".
+constexpr int32_t kIdNone = 32767;
+constexpr int32_t kIdEscape = -32768;
+
 struct ColorInfo {
     uint32_t slotOffset;  // first slot of the species in pos/img/flags
     uint32_t count;       // monomers of the species
     uint32_t nbrOffset;   // first element of its neighbour matrix
     uint32_t nbrPitch;    // elements per neighbour row (>= count, padded)
     uint32_t maxDegree;   // rows of the neighbour matrix
-    uint32_t pad_[3];
+    uint32_t idsOffset;   // first word of its members' compressed partner ids
+    uint32_t idWords;     // 32-bit words per member there (two 16-bit ids each)
+    uint32_t pad_;
 };
 
 // Accepted moves are not counted with atomics: workgroup w of the pass-2
@@ -231,6 +237,8 @@ struct SysDesc {
     DevState* state;
     uint2* list;             // pass-1 survivors, one list per pass-1 workgroup
     uint32_t* listCount;     // entries of each list
+    const uint32_t* ids;     // compressed partner ids (see layout.h)
+    const uint32_t* rowBase; // kMaxConnectivity per species: base slot of each row's deltas
     uint64_t seed;           // key of the substep draw
     uint32_t mask[3];        // Z-curve bit masks of x, y, z
     uint32_t nColors;
@@ -239,7 +247,7 @@ struct SysDesc {
     uint32_t maxColorCount;  // largest species
     uint32_t maxDegree;      // largest bond count of any monomer
     uint32_t accEntries;     // entries of state->acceptedByBlock
-    uint32_t pad_;
+    uint32_t nSlots;         // slots of pos (decoded ids are clamped below it)
 };
 
 }  // namespace pscbfm

@@ -34,7 +34,7 @@ namespace {
 
 uint64_t align_up(uint64_t v, uint64_t a) { return (v + a - 1) / a * a; }
 
-enum { P_POS, P_IMGX, P_IMGY, P_IMGZ, P_LIST, P_LCOUNT, P_NBR, P_LAT, P_PROP, P_COLORS, P_BOND, P_STATE, P_ACC, P_N };
+enum { P_POS, P_IMGX, P_IMGY, P_IMGZ, P_LIST, P_LCOUNT, P_NBR, P_LAT, P_PROP, P_COLORS, P_BOND, P_STATE, P_ACC, P_IDS, P_ROWB, P_N };
 
 struct Plan {
     uint64_t bytes[P_N];
@@ -60,6 +60,8 @@ Plan plan(const SystemLayout& L, uint32_t latticeKind, bool withLists, uint32_t 
     p.bytes[P_BOND] = 64;
     p.bytes[P_STATE] = sizeof(DevState);
     p.bytes[P_ACC] = std::max<uint64_t>(accEntries, 1) * sizeof(uint64_t);
+    p.bytes[P_IDS] = std::max<uint64_t>(L.ids.size(), 1) * 4;
+    p.bytes[P_ROWB] = std::max<uint64_t>(L.rowBase.size(), 1) * 4;
     p.total = 0;
     for (int i = 0; i < P_N; ++i) {
         p.off[i] = p.total;
@@ -97,6 +99,9 @@ void DeviceSystem::allocate(const SystemLayout& L, uint32_t latticeKind, uint64_
     desc.proposal = reinterpret_cast<uint32_t*>(base + p.off[P_PROP]);
     desc.colors = reinterpret_cast<const ColorInfo*>(base + p.off[P_COLORS]);
     desc.bondMask = reinterpret_cast<const uint32_t*>(base + p.off[P_BOND]);
+    desc.ids = reinterpret_cast<const uint32_t*>(base + p.off[P_IDS]);
+    desc.rowBase = reinterpret_cast<const uint32_t*>(base + p.off[P_ROWB]);
+    desc.nSlots = std::max<uint32_t>(L.nSlots, 1u);
     desc.state = reinterpret_cast<DevState*>(base + p.off[P_STATE]);
     desc.seed = seed;
     for (int a = 0; a < 3; ++a) desc.mask[a] = L.codec.mask[a];
@@ -119,6 +124,8 @@ void DeviceSystem::allocate(const SystemLayout& L, uint32_t latticeKind, uint64_
     up(P_NBR, L.nbr.data(), uint64_t(L.nbr.size()) * 4);
     up(P_COLORS, L.colors.data(), uint64_t(L.colors.size()) * sizeof(ColorInfo));
     up(P_BOND, L.bondMask, 64);
+    up(P_IDS, L.ids.data(), uint64_t(L.ids.size()) * 4);
+    up(P_ROWB, L.rowBase.data(), uint64_t(L.rowBase.size()) * 4);
     // Host staging buffers live until the stream synchronizes below: the
     // copies are asynchronous.
     std::vector<uint8_t> latBytes;

@@ -114,6 +114,8 @@ __device__ __forceinline__ ColorInfo load_color(const ColorInfo* colors, uint32_
     c.nbrOffset = w[offsetof(ColorInfo, nbrOffset) / 4];
     c.nbrPitch = w[offsetof(ColorInfo, nbrPitch) / 4];
     c.maxDegree = w[offsetof(ColorInfo, maxDegree) / 4];
+    c.idsOffset = w[offsetof(ColorInfo, idsOffset) / 4];
+    c.idWords = w[offsetof(ColorInfo, idWords) / 4];
     return c;
 }
 
@@ -207,6 +209,14 @@ __device__ __forceinline__ void check_body(const SysDesc& d, uint64_t t, Substep
     const gptr<const uint32_t> posAll = global_ptr(static_cast<const uint32_t*>(d.pos));
     const gptr<const uint32_t> pos = posAll + ci.slotOffset;
     const gptr<const uint32_t> nbr = global_ptr(d.nbr);
+    // Partner ids: the bit-lattice kernel reads the full matrix rows; the
+    // uint8-lattice kernel, whose plane words cost more traffic, reads the
+    // 16-bit codes (half the bytes) and the full matrix only for escapes.
+    const gptr<const uint32_t> ids = global_ptr(d.ids) + ci.idsOffset;
+    uint32_t rb[MAXDEG];
+#pragma unroll
+    for (int r = 0; r < MAXDEG; ++r)
+        rb[r] = LAT == kLatticeU8 ? global_ptr(d.rowBase)[draw.species * kMaxConnectivity + uint32_t(r)] : 0u;
     const gptr<const uint64_t> lattice = global_ptr(static_cast<const uint64_t*>(d.lattice));
 
     // The load phase is branch-free: lanes without a monomer (or without a
@@ -222,11 +232,31 @@ __device__ __forceinline__ void check_body(const SysDesc& d, uint64_t t, Substep
         live[s] = j[s] < ci.count;
         const uint32_t jj = live[s] ? j[s] : 0u;  // count > 0 for every species
         m[s] = pos[jj];
+        if constexpr (LAT == kLatticeU8) {
 #pragma unroll
-        for (int r = 0; r < MAXDEG; ++r) {
-            const bool row = uint32_t(r) < ci.maxDegree;
-            const uint32_t id = nbr[row ? ci.nbrOffset + size_t(r) * ci.nbrPitch + jj : 0u];
-            partner[s][r] = (live[s] && row) ? id : kNoNeighbor;
+            for (int q = 0; q < MAXDEG / 2; ++q) {
+                const bool have = uint32_t(q) < ci.idWords;  // the same for every lane
+                const uint32_t w = ids[have ? size_t(jj) * ci.idWords + uint32_t(q) : 0u];
+#pragma unroll
+                for (int h = 0; h < 2; ++h) {
+                    const int r = 2 * q + h;
+                    const int32_t v = int32_t(int16_t(uint16_t(h ? (w >> 16) : (w & 0xFFFFu))));
+                    uint32_t id = rb[r] + jj + uint32_t(v);
+                    // an escape is read only for a real row of this species, so the index stays in its matrix
+                    if (have && v == kIdEscape && uint32_t(r) < ci.maxDegree)
+                        id = nbr[ci.nbrOffset + size_t(r) * ci.nbrPitch + jj];
+                    id = min(id, d.nSlots - 1u);  // a valid slot whatever the code
+                    const bool has = have && v != kIdNone && uint32_t(r) < ci.maxDegree;
+                    partner[s][r] = (live[s] && has) ? id : kNoNeighbor;
+                }
+            }
+        } else {
+#pragma unroll
+            for (int r = 0; r < MAXDEG; ++r) {
+                const bool row = uint32_t(r) < ci.maxDegree;
+                const uint32_t id = nbr[row ? ci.nbrOffset + size_t(r) * ci.nbrPitch + jj : 0u];
+                partner[s][r] = (live[s] && row) ? id : kNoNeighbor;
+            }
         }
     }
     // Second batch: partner codes and the front plane.  CRITICAL.cu:322-323: the

@@ -22,6 +22,60 @@ uint32_t bond_key_to_reference_index(uint32_t key) {
     return dx | (dy << 3) | (dz << 6);
 }
 
+namespace {
+
+// 16-bit partner codes, two per word.  Every member of a species has W codes,
+// W the system's largest degree rounded up to 2, 4 or 8.  The code of row r of
+// member j in species c is the partner's slot minus (rowBase[c][r] + j), where
+// rowBase[c][r] is the first slot of the species most row-r partners of c
+// belong to: chains and networks built in order keep a partner near the same
+// rank in its own species, so the difference is small.  A difference outside
+// +-32766 stores kIdEscape (the kernel then reads the full matrix), a missing
+// partner kIdNone; padding members hold kIdNone only.
+void encode_partner_ids(const SystemInput& in, SystemLayout& L) {
+    const uint32_t n = L.nMonomers;
+    const uint32_t nColors = uint32_t(L.colors.size());
+    uint32_t sysMaxDeg = 0;
+    for (const ColorInfo& ci : L.colors) sysMaxDeg = std::max(sysMaxDeg, ci.maxDegree);
+    const uint32_t W = sysMaxDeg <= 2u ? 2u : (sysMaxDeg <= 4u ? 4u : 8u);
+
+    std::vector<uint32_t> votes(size_t(nColors) * kMaxConnectivity * nColors, 0u);
+    for (uint32_t i = 0; i < n; ++i)
+        for (uint32_t k = 0; k < in.bonds.degree(i); ++k)
+            ++votes[(size_t(L.color[i]) * kMaxConnectivity + k) * nColors + L.color[in.bonds.neighbor(i, k)]];
+    L.rowBase.assign(size_t(nColors) * kMaxConnectivity, 0u);
+    for (uint32_t c = 0; c < nColors; ++c)
+        for (uint32_t r = 0; r < kMaxConnectivity; ++r) {
+            const uint32_t* v = &votes[(size_t(c) * kMaxConnectivity + r) * nColors];
+            const uint32_t best = uint32_t(std::max_element(v, v + nColors) - v);
+            L.rowBase[size_t(c) * kMaxConnectivity + r] = L.colors[best].slotOffset;
+        }
+
+    const uint32_t none = uint32_t(kIdNone) & 0xFFFFu;
+    const uint32_t escape = uint32_t(kIdEscape) & 0xFFFFu;
+    uint32_t words = 0;
+    for (ColorInfo& ci : L.colors) {
+        ci.idsOffset = words;
+        ci.idWords = W / 2;
+        words += ci.nbrPitch * (W / 2);
+    }
+    L.ids.assign(std::max<uint32_t>(words, 1u), none | (none << 16));
+    for (uint32_t i = 0; i < n; ++i) {
+        const uint32_t c = L.color[i];
+        const ColorInfo& ci = L.colors[c];
+        const uint32_t j = L.slotOf[i] - ci.slotOffset;
+        for (uint32_t r = 0; r < in.bonds.degree(i); ++r) {
+            const int64_t v = int64_t(L.slotOf[in.bonds.neighbor(i, r)]) -
+                              int64_t(L.rowBase[size_t(c) * kMaxConnectivity + r]) - int64_t(j);
+            const uint32_t code = (v > 32766 || v < -32767) ? escape : (uint32_t(int32_t(v)) & 0xFFFFu);
+            uint32_t& w = L.ids[ci.idsOffset + size_t(j) * (W / 2) + r / 2];
+            w = (r & 1u) ? ((w & 0xFFFFu) | (code << 16)) : ((w & 0xFFFF0000u) | code);
+        }
+    }
+}
+
+}  // namespace
+
 SystemLayout build_layout(const SystemInput& in, const std::vector<uint8_t>* colorOverride,
                           bool uniformColors) {
     SystemLayout L;
@@ -92,6 +146,8 @@ SystemLayout build_layout(const SystemInput& in, const std::vector<uint8_t>* col
         for (uint32_t k = 0; k < in.bonds.degree(i); ++k)
             L.nbr[ci.nbrOffset + k * ci.nbrPitch + j] = L.slotOf[in.bonds.neighbor(i, k)];
     }
+
+    encode_partner_ids(in, L);
 
     // Positions: wrapped corner as Z-curve code, plus image counters so the
     // unfolded coordinate can be rebuilt.

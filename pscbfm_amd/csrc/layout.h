@@ -28,6 +28,12 @@ struct SystemLayout {
     std::vector<uint32_t> pos;           // per slot, Z-curve code
     std::vector<int32_t> img[3];         // per slot, periodic images
     std::vector<uint32_t> nbr;           // neighbour matrices (slot ids)
+    // Compressed partner ids: member j of species c has W 16-bit codes at
+    // ids[colors[c].idsOffset + j*W/2 ...], two per word, W the system's
+    // degree bucket.  Code v of row r means slot rowBase[c*8 + r] + j + v;
+    // kIdNone: no partner; kIdEscape: read nbr (the full matrix) instead.
+    std::vector<uint32_t> ids;
+    std::vector<uint32_t> rowBase;
     uint32_t bondMask[16];               // Z-order bond keys, bit set = forbidden
     uint32_t maxColorCount = 0;
 

@@ -73,3 +73,17 @@ def test_largest_box_across_the_wrapping_corner():
     assert np.array_equal(glog, olog)
     assert np.array_equal(gx, ox)
     assert glog.sum() > 0
+
+
+@pytest.mark.parametrize("kind", KINDS)
+def test_crosslinked_network_with_scattered_partners(kind):
+    # 1000 four-arm stars grown into a connected network: a large share of the
+    # bonds join monomers whose slots lie far from the row's common partner
+    # base, so the uint8 kernel's 16-bit partner codes fall back to the full
+    # neighbour matrix for them
+    s = stars((128, 128, 128), 1000, 4, 29, seed=7)
+    print(f"network kind {kind}: {s.n} monomers", flush=True)
+    gx, glog, ox, olog = gpu_vs_oracle(s, seed=17, n_mcs=30, kind=kind)
+    assert np.array_equal(glog, olog)
+    assert np.array_equal(gx, ox)
+    assert glog.sum() > 0
