@@ -12,6 +12,8 @@
 //   bond     64 bytes             forbidden bond keys (512 bits)
 //   state    DevState
 //   acc      uint64 per workgroup accepted-move running totals
+//   ids      16-bit partner codes, two per word (uint8 lattice only)
+//   rowBase  uint32 per (species, row): base slot of the codes (uint8 lattice only)
 #include "device_system.h"
 
 #include <algorithm>
@@ -60,8 +62,10 @@ Plan plan(const SystemLayout& L, uint32_t latticeKind, bool withLists, uint32_t 
     p.bytes[P_BOND] = 64;
     p.bytes[P_STATE] = sizeof(DevState);
     p.bytes[P_ACC] = std::max<uint64_t>(accEntries, 1) * sizeof(uint64_t);
-    p.bytes[P_IDS] = std::max<uint64_t>(L.ids.size(), 1) * 4;
-    p.bytes[P_ROWB] = std::max<uint64_t>(L.rowBase.size(), 1) * 4;
+    // only the uint8-lattice kernel reads the 16-bit partner codes
+    const bool codes = latticeKind == kLatticeU8;
+    p.bytes[P_IDS] = codes ? std::max<uint64_t>(L.ids.size(), 1) * 4 : 4;
+    p.bytes[P_ROWB] = codes ? std::max<uint64_t>(L.rowBase.size(), 1) * 4 : 4;
     p.total = 0;
     for (int i = 0; i < P_N; ++i) {
         p.off[i] = p.total;
@@ -124,8 +128,10 @@ void DeviceSystem::allocate(const SystemLayout& L, uint32_t latticeKind, uint64_
     up(P_NBR, L.nbr.data(), uint64_t(L.nbr.size()) * 4);
     up(P_COLORS, L.colors.data(), uint64_t(L.colors.size()) * sizeof(ColorInfo));
     up(P_BOND, L.bondMask, 64);
-    up(P_IDS, L.ids.data(), uint64_t(L.ids.size()) * 4);
-    up(P_ROWB, L.rowBase.data(), uint64_t(L.rowBase.size()) * 4);
+    if (latticeKind == kLatticeU8) {
+        up(P_IDS, L.ids.data(), uint64_t(L.ids.size()) * 4);
+        up(P_ROWB, L.rowBase.data(), uint64_t(L.rowBase.size()) * 4);
+    }
     // Host staging buffers live until the stream synchronizes below: the
     // copies are asynchronous.
     std::vector<uint8_t> latBytes;
