@@ -19,6 +19,7 @@
 #include <algorithm>
 #include <cstring>
 #include <limits>
+#include <memory>
 #include <sstream>
 #include <stdexcept>
 
@@ -188,17 +189,19 @@ uint64_t DeviceSystem::acceptedTotal(uint32_t entries, hipStream_t stream) const
 
 void DeviceSystem::positions(const SystemLayout& L, int32_t* xyzOut, hipStream_t stream) const {
     const uint32_t ns = L.nSlots;
-    std::vector<uint32_t> pos(ns);
-    std::vector<int32_t> img[3];
-    for (auto& v : img) v.resize(ns);
-    if (ns) {
-        hip_check(hipMemcpyAsync(pos.data(), desc.pos, size_t(ns) * 4, hipMemcpyDeviceToHost, stream), "D2H");
-        hip_check(hipMemcpyAsync(img[0].data(), desc.imgX, size_t(ns) * 4, hipMemcpyDeviceToHost, stream), "D2H");
-        hip_check(hipMemcpyAsync(img[1].data(), desc.imgY, size_t(ns) * 4, hipMemcpyDeviceToHost, stream), "D2H");
-        hip_check(hipMemcpyAsync(img[2].data(), desc.imgZ, size_t(ns) * 4, hipMemcpyDeviceToHost, stream), "D2H");
-        hip_check(hipStreamSynchronize(stream), "sync");
-    }
-    L.unfolded(pos, img, xyzOut);
+    if (!ns) return;
+    // pos, imgX, imgY and imgZ are consecutive pieces of the arena: one copy
+    const char* first = reinterpret_cast<const char*>(desc.pos);
+    const uint64_t span = uint64_t(reinterpret_cast<const char*>(desc.imgZ) - first) + uint64_t(ns) * 4;
+    std::unique_ptr<uint32_t[]> host(new uint32_t[(span + 3) / 4]);  // no zero fill: the copy writes it all
+    hip_check(hipMemcpyAsync(host.get(), first, span, hipMemcpyDeviceToHost, stream), "positions D2H");
+    hip_check(hipStreamSynchronize(stream), "sync");
+    const char* h = reinterpret_cast<const char*>(host.get());
+    auto at = [&](const void* dev) { return h + (reinterpret_cast<const char*>(dev) - first); };
+    const int32_t* img[3] = {reinterpret_cast<const int32_t*>(at(desc.imgX)),
+                             reinterpret_cast<const int32_t*>(at(desc.imgY)),
+                             reinterpret_cast<const int32_t*>(at(desc.imgZ))};
+    L.unfolded(reinterpret_cast<const uint32_t*>(h), img, xyzOut);
 }
 
 }  // namespace pscbfm

@@ -8,6 +8,7 @@
 #include <cstring>
 #include <stdexcept>
 #include <string>
+#include <thread>
 
 namespace pscbfm {
 
@@ -171,16 +172,45 @@ SystemLayout build_layout(const SystemInput& in, const std::vector<uint8_t>* col
     return L;
 }
 
-void SystemLayout::unfolded(const std::vector<uint32_t>& posSlots, const std::vector<int32_t>* imgSlots,
-                            int32_t* xyzOut) const {
-    const uint32_t box[3] = {1u << codec.log2L[0], 1u << codec.log2L[1], 1u << codec.log2L[2]};
-    for (uint32_t i = 0; i < nMonomers; ++i) {
-        const uint32_t s = slotOf[i];
-        uint32_t w[3];
-        codec.decode(posSlots[s], &w[0], &w[1], &w[2]);
-        for (int a = 0; a < 3; ++a)
-            xyzOut[3 * size_t(i) + a] = int32_t(w[a]) + imgSlots[a][s] * int32_t(box[a]);
+void SystemLayout::unfolded(const uint32_t* posSlots, const int32_t* const imgSlots[3], int32_t* xyzOut) const {
+    const int32_t box[3] = {int32_t(1u << codec.log2L[0]), int32_t(1u << codec.log2L[1]),
+                            int32_t(1u << codec.log2L[2])};
+    // Byte-wise decode: each code bit belongs to one axis and lands on one bit
+    // of it, so an axis value is the OR of the four bytes' contributions.
+    // Each table is built from its eight single-bit entries.
+    uint16_t lut[3][4][256];
+    for (int a = 0; a < 3; ++a)
+        for (uint32_t k = 0; k < 4; ++k) {
+            uint16_t bit[8];
+            for (uint32_t j = 0; j < 8; ++j) bit[j] = uint16_t(MortonCodec::extract(1u << (8u * k + j), codec.mask[a]));
+            lut[a][k][0] = 0;
+            for (uint32_t b = 1; b < 256; ++b)
+                lut[a][k][b] = uint16_t(lut[a][k][b & (b - 1u)] | bit[__builtin_ctz(b)]);
+        }
+    auto decode = [&](uint32_t lo, uint32_t hi) {
+        for (uint32_t i = lo; i < hi; ++i) {
+            const uint32_t s = slotOf[i];
+            const uint32_t m = posSlots[s];
+            const uint32_t b0 = m & 0xFFu, b1 = (m >> 8) & 0xFFu, b2 = (m >> 16) & 0xFFu, b3 = m >> 24;
+            for (int a = 0; a < 3; ++a) {
+                const int32_t w = int32_t(lut[a][0][b0] | lut[a][1][b1] | lut[a][2][b2] | lut[a][3][b3]);
+                xyzOut[3 * size_t(i) + a] = w + imgSlots[a][s] * box[a];
+            }
+        }
+    };
+    // large systems are split over a few threads (the decode is memory-bound)
+    const uint32_t per = 1u << 20;
+    const uint32_t nThreads = std::min<uint32_t>((nMonomers + per - 1) / per, 8u);
+    if (nThreads <= 1) {
+        decode(0, nMonomers);
+        return;
     }
+    std::vector<std::thread> pool;
+    const uint32_t chunk = (nMonomers + nThreads - 1) / nThreads;
+    for (uint32_t t = 1; t < nThreads; ++t)
+        pool.emplace_back(decode, std::min(nMonomers, t * chunk), std::min(nMonomers, (t + 1) * chunk));
+    decode(0, std::min(nMonomers, chunk));
+    for (std::thread& th : pool) th.join();
 }
 
 }  // namespace pscbfm

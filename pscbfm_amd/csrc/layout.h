@@ -37,9 +37,9 @@ struct SystemLayout {
     uint32_t bondMask[16];               // Z-order bond keys, bit set = forbidden
     uint32_t maxColorCount = 0;
 
-    // Unfolded coordinates of all monomers in original order.
-    void unfolded(const std::vector<uint32_t>& posSlots, const std::vector<int32_t>* imgSlots,
-                  int32_t* xyzOut) const;
+    // Unfolded coordinates of all monomers in original order, from per-slot
+    // codes and image counters (nSlots entries each).
+    void unfolded(const uint32_t* posSlots, const int32_t* const imgSlots[3], int32_t* xyzOut) const;
 };
 
 // Floor division / modulo by a power-of-two box edge.
