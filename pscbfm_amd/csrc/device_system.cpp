@@ -193,10 +193,23 @@ void DeviceSystem::positions(const SystemLayout& L, int32_t* xyzOut, hipStream_t
     // pos, imgX, imgY and imgZ are consecutive pieces of the arena: one copy
     const char* first = reinterpret_cast<const char*>(desc.pos);
     const uint64_t span = uint64_t(reinterpret_cast<const char*>(desc.imgZ) - first) + uint64_t(ns) * 4;
-    std::unique_ptr<uint32_t[]> host(new uint32_t[(span + 3) / 4]);  // no zero fill: the copy writes it all
-    hip_check(hipMemcpyAsync(host.get(), first, span, hipMemcpyDeviceToHost, stream), "positions D2H");
+    // Staging: a per-thread buffer reused across calls up to 64 MiB (no page
+    // faults inside the copy for the many small systems of a batch); larger
+    // systems take a one-off buffer without zero fill.
+    static thread_local std::vector<uint32_t> reused;
+    std::unique_ptr<uint32_t[]> oneOff;
+    const uint64_t words = (span + 3) / 4;
+    uint32_t* host;
+    if (span <= (uint64_t(64) << 20)) {
+        if (reused.size() < words) reused.resize(words);
+        host = reused.data();
+    } else {
+        oneOff.reset(new uint32_t[words]);
+        host = oneOff.get();
+    }
+    hip_check(hipMemcpyAsync(host, first, span, hipMemcpyDeviceToHost, stream), "positions D2H");
     hip_check(hipStreamSynchronize(stream), "sync");
-    const char* h = reinterpret_cast<const char*>(host.get());
+    const char* h = reinterpret_cast<const char*>(host);
     auto at = [&](const void* dev) { return h + (reinterpret_cast<const char*>(dev) - first); };
     const int32_t* img[3] = {reinterpret_cast<const int32_t*>(at(desc.imgX)),
                              reinterpret_cast<const int32_t*>(at(desc.imgY)),
