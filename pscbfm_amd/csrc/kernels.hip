@@ -129,14 +129,29 @@ struct Move {
     bool wraps;         // the axis coordinate crosses the periodic boundary
 };
 
+// The Z-curve masks of x, y and z as values in scalar registers.  Passing
+// the descriptor's array instead lets the compiler turn the per-lane choice
+// of an axis into a per-lane load from the argument segment, one more
+// dependent round trip in front of the lattice reads.
+struct AxisMasks {
+    uint32_t x, y, z;
+};
+__device__ __forceinline__ AxisMasks axis_masks(const SysDesc& d) {
+    AxisMasks m;
+    m.x = uint32_t(__builtin_amdgcn_readfirstlane(d.mask[0]));
+    m.y = uint32_t(__builtin_amdgcn_readfirstlane(d.mask[1]));
+    m.z = uint32_t(__builtin_amdgcn_readfirstlane(d.mask[2]));
+    return m;
+}
+
 // Direction d: 0:-x 1:+x 2:-y 3:+y 4:-z 5:+z (UpdaterGPUScBFM_AB_Type.cu:534-545).
 // Front plane: p + 2 e_d + a e_u + b e_v, a,b in {-1,0,1} (CRITICAL.cu:62-176).
-__device__ __forceinline__ Move make_move(uint32_t m, uint32_t dir, const uint32_t mask[3]) {
+__device__ __forceinline__ Move make_move(uint32_t m, uint32_t dir, const AxisMasks& mask) {
     const uint32_t axis = dir >> 1;
     const bool plus = (dir & 1u) != 0;
-    const uint32_t Ma = axis == 0 ? mask[0] : (axis == 1 ? mask[1] : mask[2]);
-    const uint32_t Mu = axis == 0 ? mask[1] : mask[0];
-    const uint32_t Mv = axis == 2 ? mask[1] : mask[2];
+    const uint32_t Ma = axis == 0 ? mask.x : (axis == 1 ? mask.y : mask.z);
+    const uint32_t Mu = axis == 0 ? mask.y : mask.x;
+    const uint32_t Mv = axis == 2 ? mask.y : mask.z;
     const uint32_t oneA = Ma & (~Ma + 1u);
     const uint32_t stepA = plus ? oneA : Ma;  // +1 or -1 in dilated form
     Move mv;
@@ -218,6 +233,7 @@ __device__ __forceinline__ void check_body(const SysDesc& d, uint64_t t, Substep
     for (int r = 0; r < MAXDEG; ++r)
         rb[r] = LAT == kLatticeU8 ? global_ptr(d.rowBase)[draw.species * kMaxConnectivity + uint32_t(r)] : 0u;
     const gptr<const uint64_t> lattice = global_ptr(static_cast<const uint64_t*>(d.lattice));
+    const AxisMasks masks = axis_masks(d);
 
     // The load phase is branch-free: lanes without a monomer (or without a
     // partner in a row) load from an in-bounds stand-in address and discard
@@ -265,7 +281,7 @@ __device__ __forceinline__ void check_body(const SysDesc& d, uint64_t t, Substep
 #pragma unroll
     for (int s = 0; s < SLOTS; ++s) {
         dir[s] = move_hash(move_hash(j[s]) ^ draw.seed) % 6u;
-        const Move mv = make_move(m[s], dir[s], d.mask);
+        const Move mv = make_move(m[s], dir[s], masks);
         to[s] = mv.to;
 #pragma unroll
         for (int r = 0; r < MAXDEG; ++r)
@@ -341,6 +357,7 @@ __device__ __forceinline__ void perform_body(const SysDesc& d, const Launch& L, 
             prev[2 * size_t(q) + 1] = 0u;
         }
     }
+    const AxisMasks masks = axis_masks(d);
     const gptr<const uint64_t> T64 =
         (gptr<const uint64_t>)(global_ptr(static_cast<const uint32_t*>(d.proposal)) + size_t(t & 1u) * d.latWords);
     const gptr<uint32_t> pos = global_ptr(d.pos);
@@ -354,7 +371,7 @@ __device__ __forceinline__ void perform_body(const SysDesc& d, const Launch& L, 
         const uint32_t jj = uint32_t(e) & ((1u << kDirShift) - 1u);
         const uint32_t dir = uint32_t(e) >> kDirShift;
         const uint32_t m = uint32_t(e >> 32);
-        const Move mv = make_move(m, dir, d.mask);
+        const Move mv = make_move(m, dir, masks);
         if (plane_any<6>(T64, mv)) continue;
         const uint32_t slot = ci.slotOffset + jj;
         pos[slot] = mv.to;

@@ -16,9 +16,10 @@ pytestmark = pytest.mark.gpu
 EXE = Path(P.__file__).parent / "pscbfm_run"
 
 
-def _updater_from(path, seed):
+def _updater_from(path, seed, kind=P.LATTICE_BIT):
     u = P.UpdaterGPUScBFM_AB_Type()
     u.setGpu(0)
+    u.setLatticeKind(kind)
     mcs = u.loadBfm(path)
     u.setSeed(seed)
     u.setStartMcs(mcs)  # the simulator continues the draws from the frame's age
@@ -26,18 +27,19 @@ def _updater_from(path, seed):
     return u, mcs
 
 
-def test_cli_frames_equal_the_updater(tmp_path):
+@pytest.mark.parametrize("lattice,kind", [("bit", P.LATTICE_BIT), ("u8", P.LATTICE_U8)])
+def test_cli_frames_equal_the_updater(tmp_path, lattice, kind):
     s = chains((64, 64, 64), 40, 64, seed=11)
     src = tmp_path / "in.bfm"
     write_bfm(src, s.box, s.xyz, s.bonds, mcs=100)
     out = tmp_path / "out.bfm"
-    print("cli run", flush=True)
+    print(f"cli run, lattice {lattice}", flush=True)
     res = subprocess.run([str(EXE), "-i", str(src), "-m", "20", "-s", "10", "-o", str(out), "-g", "0",
-                          "--seed", "42"], capture_output=True, text=True, timeout=120)
+                          "--seed", "42", "--lattice", lattice], capture_output=True, text=True, timeout=120)
     assert res.returncode == 0, res.stderr
     got = read_bfm(out)
     assert [f[0] for f in got.frames] == [100, 110, 120]
-    u, mcs0 = _updater_from(src, 42)
+    u, mcs0 = _updater_from(src, 42, kind)
     assert mcs0 == 100 and np.array_equal(got.frames[0][1], s.xyz)
     for k in (1, 2):
         u.run(10)

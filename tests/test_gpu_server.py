@@ -12,8 +12,8 @@ from helpers import chains, make_updater, oracle_for, ordered_chains, stage_upda
 pytestmark = pytest.mark.gpu
 
 
-def _alone(s, seed, mcs):
-    u = make_updater(s, seed)
+def _alone(s, seed, mcs, kind=P.LATTICE_BIT):
+    u = make_updater(s, seed, kind)
     colors = u.colors(s.n)
     u.run(mcs)
     x = u.getPositions(s.n)
@@ -40,9 +40,11 @@ def _jobs():
     ]
 
 
-def test_each_served_job_equals_its_run_alone():
+@pytest.mark.parametrize("kind", [P.LATTICE_BIT, P.LATTICE_U8])
+def test_each_served_job_equals_its_run_alone(kind):
     jobs = _jobs()
-    srv = P.Server(hbm_budget=1 << 34, tile_capacity=256, replica_capacity=16, quantum=8)
+    print(f"served jobs, lattice kind {kind}", flush=True)
+    srv = P.Server(hbm_budget=1 << 34, tile_capacity=256, replica_capacity=16, lattice_kind=kind, quantum=8)
     ids = []
     # submit from two threads while the loop is already running
     def submit(part):
@@ -58,7 +60,7 @@ def test_each_served_job_equals_its_run_alone():
     for job, s, seed, mcs in ids:
         info = srv.job(job)
         assert info.state == P.JOB_DONE and info.t_submit <= info.t_admit <= info.t_done
-        x_alone, acc_alone, colors = _alone(s, seed, mcs)
+        x_alone, acc_alone, colors = _alone(s, seed, mcs, kind)
         x_cpu, acc_cpu = _cpu(s, seed, mcs, colors)
         served = srv.result(job, s.n)
         assert np.array_equal(served, x_alone) and np.array_equal(served, x_cpu)
