@@ -25,6 +25,58 @@ uint32_t bond_key_to_reference_index(uint32_t key) {
 
 namespace {
 
+// Byte tables of a Z-curve codec.  Each code bit belongs to one axis and
+// stands for one bit of it, so a code is the OR of the deposits of the
+// coordinate bytes, and an axis value the OR of the extracts of the code
+// bytes.  Every table is built from its eight single-bit entries.
+struct ZTables {
+    uint32_t dep[3][2][256];  // coordinate byte k of axis a -> code bits (coordinates < 2^16)
+    uint16_t ext[3][4][256];  // code byte k -> bits of axis a
+    explicit ZTables(const MortonCodec& c) {
+        for (int a = 0; a < 3; ++a) {
+            for (uint32_t k = 0; k < 2; ++k) {
+                uint32_t bit[8];
+                for (uint32_t j = 0; j < 8; ++j) bit[j] = MortonCodec::deposit(1u << (8u * k + j), c.mask[a]);
+                dep[a][k][0] = 0;
+                for (uint32_t b = 1; b < 256; ++b) dep[a][k][b] = dep[a][k][b & (b - 1u)] | bit[__builtin_ctz(b)];
+            }
+            for (uint32_t k = 0; k < 4; ++k) {
+                uint16_t bit[8];
+                for (uint32_t j = 0; j < 8; ++j) bit[j] = uint16_t(MortonCodec::extract(1u << (8u * k + j), c.mask[a]));
+                ext[a][k][0] = 0;
+                for (uint32_t b = 1; b < 256; ++b)
+                    ext[a][k][b] = uint16_t(ext[a][k][b & (b - 1u)] | bit[__builtin_ctz(b)]);
+            }
+        }
+    }
+    uint32_t encode(uint32_t x, uint32_t y, uint32_t z) const {
+        return dep[0][0][x & 0xFFu] | dep[0][1][(x >> 8) & 0xFFu] | dep[1][0][y & 0xFFu] |
+               dep[1][1][(y >> 8) & 0xFFu] | dep[2][0][z & 0xFFu] | dep[2][1][(z >> 8) & 0xFFu];
+    }
+    uint32_t axis(uint32_t m, int a) const {
+        return uint32_t(ext[a][0][m & 0xFFu] | ext[a][1][(m >> 8) & 0xFFu] | ext[a][2][(m >> 16) & 0xFFu] |
+                        ext[a][3][m >> 24]);
+    }
+};
+
+// f(lo, hi) over [0, n): on up to 8 threads when n is large (per-monomer
+// loops here are memory-bound and independent).
+template <class F>
+void parallel_for(uint32_t n, F&& f) {
+    const uint32_t per = 1u << 20;
+    const uint32_t nThreads = std::min<uint32_t>((n + per - 1) / per, 8u);
+    if (nThreads <= 1) {
+        f(0u, n);
+        return;
+    }
+    std::vector<std::thread> pool;
+    const uint32_t chunk = (n + nThreads - 1) / nThreads;
+    for (uint32_t t = 1; t < nThreads; ++t)
+        pool.emplace_back([&f, t, chunk, n] { f(std::min(n, t * chunk), std::min(n, (t + 1) * chunk)); });
+    f(0u, std::min(n, chunk));
+    for (std::thread& th : pool) th.join();
+}
+
 // 16-bit partner codes, two per word.  Every member of a species has W codes,
 // W the system's largest degree rounded up to 2, 4 or 8.  The code of row r of
 // member j in species c is the partner's slot minus (rowBase[c][r] + j), where
@@ -154,17 +206,20 @@ SystemLayout build_layout(const SystemInput& in, const std::vector<uint8_t>* col
     // unfolded coordinate can be rebuilt.
     L.pos.assign(L.nSlots, 0);
     for (int a = 0; a < 3; ++a) L.img[a].assign(L.nSlots, 0);
-    for (uint32_t i = 0; i < n; ++i) {
-        const uint32_t s = L.slotOf[i];
-        uint32_t w[3];
-        for (int a = 0; a < 3; ++a) {
-            const int32_t v = in.xyz[3 * size_t(i) + a];
-            const int32_t im = floor_div_pow2(v, L.codec.log2L[a]);
-            L.img[a][s] = im;
-            w[a] = uint32_t(v - im * int32_t(in.box[a]));
+    const ZTables z(L.codec);
+    parallel_for(n, [&](uint32_t lo, uint32_t hi) {
+        for (uint32_t i = lo; i < hi; ++i) {
+            const uint32_t s = L.slotOf[i];
+            uint32_t w[3];
+            for (int a = 0; a < 3; ++a) {
+                const int32_t v = in.xyz[3 * size_t(i) + a];
+                const int32_t im = floor_div_pow2(v, L.codec.log2L[a]);
+                L.img[a][s] = im;
+                w[a] = uint32_t(v - im * int32_t(in.box[a]));
+            }
+            L.pos[s] = z.encode(w[0], w[1], w[2]);
         }
-        L.pos[s] = L.codec.encode(w[0], w[1], w[2]);
-    }
+    });
 
     std::memset(L.bondMask, 0, sizeof L.bondMask);
     for (uint32_t key = 0; key < 512; ++key)
@@ -175,42 +230,14 @@ SystemLayout build_layout(const SystemInput& in, const std::vector<uint8_t>* col
 void SystemLayout::unfolded(const uint32_t* posSlots, const int32_t* const imgSlots[3], int32_t* xyzOut) const {
     const int32_t box[3] = {int32_t(1u << codec.log2L[0]), int32_t(1u << codec.log2L[1]),
                             int32_t(1u << codec.log2L[2])};
-    // Byte-wise decode: each code bit belongs to one axis and lands on one bit
-    // of it, so an axis value is the OR of the four bytes' contributions.
-    // Each table is built from its eight single-bit entries.
-    uint16_t lut[3][4][256];
-    for (int a = 0; a < 3; ++a)
-        for (uint32_t k = 0; k < 4; ++k) {
-            uint16_t bit[8];
-            for (uint32_t j = 0; j < 8; ++j) bit[j] = uint16_t(MortonCodec::extract(1u << (8u * k + j), codec.mask[a]));
-            lut[a][k][0] = 0;
-            for (uint32_t b = 1; b < 256; ++b)
-                lut[a][k][b] = uint16_t(lut[a][k][b & (b - 1u)] | bit[__builtin_ctz(b)]);
-        }
-    auto decode = [&](uint32_t lo, uint32_t hi) {
+    const ZTables z(codec);
+    parallel_for(nMonomers, [&](uint32_t lo, uint32_t hi) {
         for (uint32_t i = lo; i < hi; ++i) {
             const uint32_t s = slotOf[i];
             const uint32_t m = posSlots[s];
-            const uint32_t b0 = m & 0xFFu, b1 = (m >> 8) & 0xFFu, b2 = (m >> 16) & 0xFFu, b3 = m >> 24;
-            for (int a = 0; a < 3; ++a) {
-                const int32_t w = int32_t(lut[a][0][b0] | lut[a][1][b1] | lut[a][2][b2] | lut[a][3][b3]);
-                xyzOut[3 * size_t(i) + a] = w + imgSlots[a][s] * box[a];
-            }
+            for (int a = 0; a < 3; ++a) xyzOut[3 * size_t(i) + a] = int32_t(z.axis(m, a)) + imgSlots[a][s] * box[a];
         }
-    };
-    // large systems are split over a few threads (the decode is memory-bound)
-    const uint32_t per = 1u << 20;
-    const uint32_t nThreads = std::min<uint32_t>((nMonomers + per - 1) / per, 8u);
-    if (nThreads <= 1) {
-        decode(0, nMonomers);
-        return;
-    }
-    std::vector<std::thread> pool;
-    const uint32_t chunk = (nMonomers + nThreads - 1) / nThreads;
-    for (uint32_t t = 1; t < nThreads; ++t)
-        pool.emplace_back(decode, std::min(nMonomers, t * chunk), std::min(nMonomers, (t + 1) * chunk));
-    decode(0, std::min(nMonomers, chunk));
-    for (std::thread& th : pool) th.join();
+    });
 }
 
 }  // namespace pscbfm
