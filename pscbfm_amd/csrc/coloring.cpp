@@ -17,15 +17,20 @@ constexpr uint8_t kUncolored = 0xFF;
 constexpr int64_t kNone = -1;
 
 // Smallest color not taken by an already colored neighbour of `node`.
+// `taken` is all zero on entry and on return: only the entries the
+// neighbours set are cleared again, not the whole table.
 uint32_t smallest_free_color(const Adjacency& adj, const std::vector<uint8_t>& color,
                              uint32_t node, std::vector<uint8_t>& taken) {
-    std::fill(taken.begin(), taken.end(), 0);
     for (uint32_t k = 0; k < adj.degree(node); ++k) {
         const uint8_t c = color[adj.neighbor(node, k)];
         if (c != kUncolored) taken[c] = 1;
     }
     uint32_t c = 0;
     while (c < kMaxColors && taken[c]) ++c;
+    for (uint32_t k = 0; k < adj.degree(node); ++k) {
+        const uint8_t t = color[adj.neighbor(node, k)];
+        if (t != kUncolored) taken[t] = 0;
+    }
     return c;
 }
 
