@@ -66,14 +66,38 @@ def test_launch_shape_does_not_change_results(block, slots):
     assert np.array_equal(gx, ox)
 
 
-def test_invariants_after_run():
+@pytest.mark.parametrize("kind", [P.LATTICE_BIT, P.LATTICE_U8])
+def test_invariants_after_run(kind):
     s = chains((64, 64, 64), 50, 50, seed=4)
-    u = make_updater(s, seed=1)
+    u = make_updater(s, seed=1, kind=kind)
     u.runSimulationOnGPU(25)
     u.checkSystem()
-    # occupancy lattice holds exactly one set cell per monomer
+    # the occupancy lattice holds exactly the wrapped corners of the
+    # positions read back, one cell per monomer
     lat = u.downloadLattice(64 ** 3)
-    assert int(lat.sum()) == s.n
+    corners = u.getPositions(s.n) % 64
+    want = np.zeros(64 ** 3, np.uint8)
+    want[[P.zcurve_encode(s.box, *c) for c in corners]] = 1
+    assert int(lat.sum()) == s.n and np.array_equal(lat, want)
+    u.cleanup()
+
+
+@pytest.mark.parametrize("kind", [P.LATTICE_BIT, P.LATTICE_U8])
+def test_proposal_lattice_after_run(kind):
+    # The reference clears its proposal lattice before every substep
+    # (CRITICAL.cu:561-573) and its legacy updater checks that it is empty
+    # after a run.  Here substep t marks buffer t & 1 and its pass 2 clears
+    # the other buffer, so after substeps 0..T-1 the buffer the next substep
+    # uses is empty and the other holds the last substep's marks: at least one
+    # per accepted move (their targets are distinct), at most one per member
+    # of the drawn species.
+    s = chains((64, 64, 64), 50, 50, seed=4)
+    u = make_updater(s, seed=1, kind=kind)
+    _, log = u.run(25, log=True)
+    steps = len(log)
+    bits = u.proposalBits()
+    assert bits[steps & 1] == 0
+    assert 0 < log[-1] <= bits[(steps - 1) & 1] <= int(max(u.colorSizes()))
     u.cleanup()
 
 
