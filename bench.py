@@ -317,7 +317,7 @@ def main() -> int:
                     help="seconds for each single-core CPU figure (0 = skip them)")
     ap.add_argument("--replicas", type=int, default=1024, help="systems of the batched comparison")
     ap.add_argument("--replica-mcs", type=int, default=500)
-    ap.add_argument("--tune-sample", type=int, default=64)
+    ap.add_argument("--tune-sample", type=int, default=128)
     ap.add_argument("--batch-block", type=int, default=0, help="fixed batch block (with --batch-slots; 0: autotune)")
     ap.add_argument("--batch-slots", type=int, default=0, help="fixed batch slots (with --batch-block; 0: autotune)")
     ap.add_argument("--tune-mcs", type=int, default=5)
