@@ -245,7 +245,7 @@ def batched_vs_plain(n_rep: int, mcs: int, seed: int, kind: int, tune_sample: in
         f"(added in {time.perf_counter() - tic:.1f} s)")
     tic = time.perf_counter()
     batch_ms = batch.run(mcs)
-    b_xyz = [batch.positions(i, r.n) for i, r in zip(ids, reps)]
+    b_xyz = batch.positions_many(ids, [r.n for r in reps])
     batch_wall = time.perf_counter() - tic
 
     if not plain:  # batch only (profiling runs)

@@ -165,6 +165,7 @@ _SIGNATURES = {
     "pscbfm_batch_run": (C.c_int, [_H, C.c_uint64, C.POINTER(C.c_double)]),
     "pscbfm_batch_info": (C.c_int, [_H, C.c_uint32, C.POINTER(ReplicaInfo)]),
     "pscbfm_batch_get_positions": (C.c_int, [_H, C.c_uint32, _i32p, C.c_uint32]),
+    "pscbfm_batch_get_positions_many": (C.c_int, [_H, _u32p, C.c_uint32, C.POINTER(_i32p), _u32p]),
     "pscbfm_batch_usage": (C.c_int, [_H, _u32p, _u32p, _u32p, _u64p]),
     "pscbfm_autotune": (C.c_int, [_H, C.c_uint32, _u32p, _u32p, C.POINTER(C.c_double)]),
     "pscbfm_batch_autotune": (C.c_int, [C.c_int, C.c_int, C.POINTER(_H), C.c_uint32, C.c_uint32, _u32p, _u32p,
@@ -530,6 +531,18 @@ class Batch:
         out = np.empty((n, 3), dtype=np.int32)
         self._call("pscbfm_batch_get_positions", int(rid), _ptr(out, C.c_int32), int(n))
         return out
+
+    def positions_many(self, rids, ns) -> list:
+        """positions() of several replicas with one synchronization."""
+        ids = np.ascontiguousarray(rids, dtype=np.uint32)
+        counts = np.ascontiguousarray(ns, dtype=np.uint32)
+        if ids.shape != counts.shape:
+            raise InvalidArgument("one count per replica id")
+        outs = [np.empty((int(n), 3), dtype=np.int32) for n in counts]
+        ptrs = (_i32p * max(len(outs), 1))(*[_ptr(o, C.c_int32) for o in outs])
+        self._call("pscbfm_batch_get_positions_many", _ptr(ids, C.c_uint32), int(len(ids)), ptrs,
+                   _ptr(counts, C.c_uint32))
+        return outs
 
     def usage(self):
         """(live replicas, used workgroups, workgroup capacity, device bytes)."""

@@ -9,9 +9,14 @@
 #include "batch.h"
 
 #include <algorithm>
+#include <atomic>
 #include <cstring>
+#include <exception>
+#include <memory>
+#include <mutex>
 #include <sstream>
 #include <stdexcept>
+#include <thread>
 
 namespace pscbfm {
 
@@ -326,6 +331,55 @@ void Batch::positions(uint32_t id, int32_t* xyz, uint32_t n) {
     if (n != r.layout.nMonomers) throw std::invalid_argument("batch: count differs from the replica's monomers");
     hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
     r.dev->positions(r.layout, xyz, stream_);
+}
+
+void Batch::positionsMany(const uint32_t* ids, uint32_t count, int32_t* const* xyz, const uint32_t* n) {
+    std::vector<uint64_t> offset(size_t(count) + 1, 0);
+    for (uint32_t k = 0; k < count; ++k) {
+        const uint32_t id = ids[k];
+        if (id >= replicaCap_ || !replicas_[id].live) throw std::invalid_argument("batch: no such replica");
+        const Replica& r = replicas_[id];
+        if (n[k] != r.layout.nMonomers) throw std::invalid_argument("batch: count differs from the replica's monomers");
+        if (!xyz[k] && n[k] != 0) throw std::invalid_argument("batch: null output");
+        offset[k + 1] = offset[k] + (r.dev->positionsBytes(r.layout) + 255u) / 256u * 256u;
+    }
+    if (count == 0) return;
+    std::unique_ptr<char[]> host(new char[offset[count]]);  // no zero fill: the copies write it all
+    hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+    for (uint32_t k = 0; k < count; ++k) {
+        const Replica& r = replicas_[ids[k]];
+        hip_check(hipMemcpyAsync(host.get() + offset[k], r.dev->positionsBase(), r.dev->positionsBytes(r.layout),
+                                 hipMemcpyDeviceToHost, stream_),
+                  "positions D2H");
+    }
+    hip_check(hipStreamSynchronize(stream_), "sync");
+    const uint32_t nThreads = std::min<uint32_t>(count, 8u);
+    std::atomic<uint32_t> next{0};
+    std::mutex failMutex;
+    std::exception_ptr failure;
+    auto work = [&] {
+        try {
+            for (uint32_t k = next++; k < count; k = next++) {
+                const Replica& r = replicas_[ids[k]];
+                r.dev->decodePositions(r.layout, host.get() + offset[k], xyz[k]);
+            }
+        } catch (...) {
+            std::lock_guard<std::mutex> lock(failMutex);
+            if (!failure) failure = std::current_exception();
+            next = count;
+        }
+    };
+    std::vector<std::thread> pool;
+    for (uint32_t t = 1; t < nThreads; ++t) {
+        try {
+            pool.emplace_back(work);
+        } catch (...) {
+            break;  // fewer decoding threads
+        }
+    }
+    work();
+    for (std::thread& th : pool) th.join();
+    if (failure) std::rethrow_exception(failure);
 }
 
 uint32_t Batch::liveReplicas() const {

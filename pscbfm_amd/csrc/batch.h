@@ -64,6 +64,9 @@ public:
 
     const BatchReplicaInfo& info(uint32_t id) const;
     void positions(uint32_t id, int32_t* xyz, uint32_t n);
+    // positions() of `count` replicas with one synchronization: every copy is
+    // queued first, then the decodes run on a few host threads.
+    void positionsMany(const uint32_t* ids, uint32_t count, int32_t* const* xyz, const uint32_t* n);
     uint32_t liveReplicas() const;
     uint32_t usedTiles() const { return usedTiles_; }
     uint32_t tileCapacity() const { return tileCap_; }

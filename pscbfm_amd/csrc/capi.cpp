@@ -669,6 +669,12 @@ int pscbfm_batch_info(pscbfm_batch* b, uint32_t id, pscbfm_replica_info* info) {
     });
 }
 
+int pscbfm_batch_get_positions_many(pscbfm_batch* b, const uint32_t* ids, uint32_t count, int32_t* const* xyz,
+                                    const uint32_t* n) {
+    if (count != 0 && (!ids || !xyz || !n)) return PSCBFM_INVALID_ARGUMENT;
+    return batch_guarded(b, [&](pscbfm::Batch& e) { e.positionsMany(ids, count, xyz, n); });
+}
+
 int pscbfm_batch_get_positions(pscbfm_batch* b, uint32_t id, int32_t* xyz, uint32_t n) {
     if (!xyz && n) return PSCBFM_INVALID_ARGUMENT;
     return batch_guarded(b, [&](pscbfm::Batch& e) { e.positions(id, xyz, n); });

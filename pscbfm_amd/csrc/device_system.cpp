@@ -209,7 +209,12 @@ void DeviceSystem::positions(const SystemLayout& L, int32_t* xyzOut, hipStream_t
     }
     hip_check(hipMemcpyAsync(host, first, span, hipMemcpyDeviceToHost, stream), "positions D2H");
     hip_check(hipStreamSynchronize(stream), "sync");
-    const char* h = reinterpret_cast<const char*>(host);
+    decodePositions(L, host, xyzOut);
+}
+
+void DeviceSystem::decodePositions(const SystemLayout& L, const void* copy, int32_t* xyzOut) const {
+    const char* first = reinterpret_cast<const char*>(desc.pos);
+    const char* h = static_cast<const char*>(copy);
     auto at = [&](const void* dev) { return h + (reinterpret_cast<const char*>(dev) - first); };
     const int32_t* img[3] = {reinterpret_cast<const int32_t*>(at(desc.imgX)),
                              reinterpret_cast<const int32_t*>(at(desc.imgY)),

@@ -40,6 +40,15 @@ struct DeviceSystem {
     uint64_t acceptedTotal(uint32_t entries, hipStream_t stream) const;
     // Unfolded coordinates in the caller's monomer order.
     void positions(const SystemLayout& L, int32_t* xyzOut, hipStream_t stream) const;
+    // The pieces positions() copies: pos, imgX, imgY and imgZ are consecutive
+    // in the arena, so one copy of positionsBytes() bytes from positionsBase()
+    // holds them; decodePositions() turns such a copy into coordinates.
+    const void* positionsBase() const { return desc.pos; }
+    uint64_t positionsBytes(const SystemLayout& L) const {
+        return uint64_t(reinterpret_cast<const char*>(desc.imgZ) - reinterpret_cast<const char*>(desc.pos)) +
+               uint64_t(L.nSlots) * 4;
+    }
+    void decodePositions(const SystemLayout& L, const void* copy, int32_t* xyzOut) const;
 };
 
 // Workgroups covering the largest species with `perGroup` monomers each.

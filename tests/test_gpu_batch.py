@@ -49,6 +49,25 @@ def test_batch_equals_each_system_alone(kind):
         assert np.array_equal(b.positions(rid, s.n), x_alone)
 
 
+@pytest.mark.parametrize("kind", [P.LATTICE_BIT, P.LATTICE_U8])
+def test_positions_many_equals_one_by_one(kind):
+    systems = _mix()
+    b = P.Batch(tile_capacity=256, replica_capacity=16, lattice_kind=kind)
+    ids = [b.add(stage_updater(s, seed, kind)) for s, seed in systems]
+    b.run(7)
+    ns = [s.n for s, _ in systems]
+    # any order and a repeated id
+    order = [ids[3], ids[0], ids[5], ids[3], ids[1], ids[2], ids[4]]
+    many = b.positions_many(order, [ns[ids.index(i)] for i in order])
+    for rid, x in zip(order, many):
+        assert np.array_equal(x, b.positions(rid, ns[ids.index(rid)]))
+    assert b.positions_many([], []) == []
+    with pytest.raises(P.InvalidArgument):
+        b.positions_many([ids[0]], [ns[0] + 1])
+    with pytest.raises(P.InvalidArgument):
+        b.positions_many([99], [1])
+
+
 def test_batch_replica_matches_oracle():
     s, seed = chains((32, 32, 32), 20, 16, seed=7), 42
     gx, glog, ox, olog = gpu_vs_oracle(s, seed=seed, n_mcs=30)
