@@ -177,13 +177,19 @@ void Server::gpuLoop() {
             }
             std::vector<std::vector<int32_t>> coords(finished.size());
             std::vector<uint64_t> accepted(finished.size());
+            std::vector<uint32_t> ids(finished.size()), counts(finished.size());
+            std::vector<int32_t*> outs(finished.size());
             for (size_t k = 0; k < finished.size(); ++k) {
-                const BatchReplicaInfo in = batch_->info(finished[k].first);
+                const BatchReplicaInfo& in = batch_->info(finished[k].first);
                 coords[k].resize(size_t(in.nMonomers) * 3);
-                batch_->positions(finished[k].first, coords[k].data(), in.nMonomers);
+                ids[k] = finished[k].first;
+                counts[k] = in.nMonomers;
+                outs[k] = coords[k].data();
                 accepted[k] = in.accepted;
-                batch_->remove(finished[k].first);
             }
+            // every finished replica read back with one synchronization
+            batch_->positionsMany(ids.data(), uint32_t(ids.size()), outs.data(), counts.data());
+            for (size_t k = 0; k < finished.size(); ++k) batch_->remove(finished[k].first);
             std::lock_guard<std::mutex> lock(mu_);
             stats_.gpuBusySeconds += busy;
             ++stats_.launches;
