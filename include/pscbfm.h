@@ -274,7 +274,8 @@ int pscbfm_batch_footprint(pscbfm_batch* b, const pscbfm_updater* staged, uint64
  * when the tile or replica capacity is exhausted. */
 int pscbfm_batch_add(pscbfm_batch* b, const pscbfm_updater* staged, uint32_t* id);
 int pscbfm_batch_remove(pscbfm_batch* b, uint32_t id);
-/* Pass-2 threads per workgroup of the batch (0 = the pass-1 block). */
+/* Pass-2 threads per workgroup of the batch (0 = the default: the pass-1
+ * block, at most 192). */
 int pscbfm_batch_set_perform_block(pscbfm_batch* b, uint32_t block);
 /* The replica stops after n_mcs more sweeps (counted from the last synchronize). */
 int pscbfm_batch_set_target(pscbfm_batch* b, uint32_t id, uint64_t n_mcs);

@@ -24,6 +24,12 @@ namespace {
 constexpr uint32_t kSpreadTiles = 64;  // replicas with at least this many workgroups use every XCD
 
 uint32_t degree_bucket(uint32_t d) { return d <= 2 ? 2u : (d <= 4 ? 4u : 8u); }
+
+// Pass-2 threads per workgroup of a batch with pass-1 block `block`: at most
+// 192.  On the 1024-replica mix (pass 1 at 256 x 2) pass 2 with 192 threads
+// took 128.6-129.3 ms for 500 MCS, with 128 130.1, with 256 131.2-134.0 and
+// with 64 137.5.
+uint32_t default_perform_block(uint32_t block) { return block > 192u ? 192u : block; }
 }  // namespace
 
 Batch::Batch(int gpu, uint32_t latticeKind, uint32_t tileCapacity, uint32_t replicaCapacity, uint32_t block,
@@ -38,6 +44,7 @@ Batch::Batch(int gpu, uint32_t latticeKind, uint32_t tileCapacity, uint32_t repl
     replicaCap_ = replicaCapacity;
     shape_.block = block;
     shape_.slots = slots;
+    shape_.performBlock = default_perform_block(block);
     shape_.grid = tileCap_;
     hip_check(hipSetDevice(gpu_), "hipSetDevice");
     hip_check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
@@ -92,7 +99,7 @@ Batch::~Batch() {
 void Batch::setPerformBlock(uint32_t block) {
     if (block != 0 && (block % 64 != 0 || block > 256)) throw std::invalid_argument("perform block must be 64..256, multiple of 64");
     hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
-    shape_.performBlock = block;
+    shape_.performBlock = block ? block : default_perform_block(shape_.block);
     dropGraphs();
 }
 
