@@ -3,20 +3,27 @@
 
 Workload (default): the dense melt, 32768 linear chains of N=256
 (8,388,608 monomers) on a 512^3 periodic lattice at occupied-site fraction
-0.5, 2 species.  A step is one Monte-Carlo step (MCS): every species drawn
-once on average, i.e. n_colors substeps of pass 1 + pass 2.
+0.5, 2 species, built by ordered packing and relaxed for 1000 MCS (part of
+the workload build, independent of --warmup).  A step is one Monte-Carlo
+step (MCS): every species drawn once on average, i.e. n_colors substeps of
+pass 1 + pass 2.
 
     python bench.py [--steps K] [--warmup W] [--lattice bit|u8]
 
 Prints ONE JSON line.  `value` = attempted monomer moves per second over the
-K timed MCS (hipEvent time of the launched substeps, inputs resident in HBM).
-Also reported:
-  roofline      pass-1 kernel: algorithmic bytes per launch / mean launch time
-                (HIP events around every pass-1 launch of a separate profiled
-                sweep), against 8 TB/s
+K timed MCS (hipEvent time of the graph replays, inputs resident in HBM,
+every graph built before the start event).  Also reported:
+  roofline      the sweep: algorithmic bytes of the K timed MCS (pass 1 +
+                pass 2, the metric's count) / their GPU time, against 8 TB/s;
+                per kernel beside it: bytes per launch over the mean launch
+                time from HIP events around every launch of a profiled sweep,
+                and over the steady-state mean of the committed rocprofv3
+                kernel trace of this workload (profiles/)
   cpu_baseline  the CPU oracle (same colored scheme, OpenMP over all host
                 threads) on a bounded sample of the same state
-  parity        the GPU re-run of the CPU sample, compared bit for bit
+  parity        the GPU re-run of the CPU sample, compared bit for bit; the
+                other lattice kind's relaxed state against this one's, and a
+                sample of it against the oracle
 """
 from __future__ import annotations
 
@@ -56,7 +63,7 @@ def algorithmic_bytes(stats, lattice_bytes_per_cell: float):
     return p1, p2
 
 
-def cpu_baseline(u, box, bonds, colors, seed, target_seconds: float):
+def cpu_baseline(u, box, bonds, colors, seed, target_seconds: float, max_mcs: int = 200):
     """Times the oracle on the current state; re-runs the same substeps on the
     GPU and compares (the oracle is the checker here, never the product)."""
     import oracle
@@ -78,7 +85,7 @@ def cpu_baseline(u, box, bonds, colors, seed, target_seconds: float):
     tic = time.perf_counter()
     probe.run(t0, n_colors, threads=threads)
     dt = time.perf_counter() - tic
-    mcs = max(1, min(200, int(target_seconds / max(dt, 1e-3))))
+    mcs = max(1, min(max_mcs, int(target_seconds / max(dt, 1e-3))))
     sysm = oracle.System(box, xyz0, start, b, colors, seed=seed)
     tic = time.perf_counter()
     _, olog = sysm.run(t0, mcs * n_colors, threads=threads)
@@ -209,7 +216,7 @@ def network_section(seed: int, mcs: int, kind: int, c_l: float) -> dict:
 
 
 def batched_vs_plain(n_rep: int, mcs: int, seed: int, kind: int, tune_sample: int, tune_mcs: int,
-                     plain: bool = True, shape=(0, 0)) -> dict:
+                     plain: bool = True, shape=(0, 0), eager_sample: int = 64) -> dict:
     """Replica mix: one batch (autotuned shape, graph-replayed launches over
     every replica) against the plain loop of one updater per system, each
     called with runSimulationOnGPU(mcs) in turn.  Every replica's final
@@ -255,6 +262,10 @@ def batched_vs_plain(n_rep: int, mcs: int, seed: int, kind: int, tune_sample: in
                 "batch_ms": batch_ms, "shape": {"block": block, "slots": slots, "workgroups": used_tiles}}
     for u in staged:
         u.initialize()
+    # The plain loop's graphs are built first, outside its timing: a caller
+    # that runs each system repeatedly builds them once.
+    for u in staged:
+        u.prepareGraphs(mcs)
     tic = time.perf_counter()
     plain_ms = 0.0
     attempted = 0
@@ -268,12 +279,32 @@ def batched_vs_plain(n_rep: int, mcs: int, seed: int, kind: int, tune_sample: in
     accepted_same = sum(int(batch.info(i).accepted == u.counters()[1]) for i, u in zip(ids, staged))
     for u in staged:
         u.cleanup()
+    # The plain loop with eager launches (one host launch per kernel, as the
+    # reference issues them), on a sample of the replicas.
+    n_eager = min(eager_sample, n_rep)
+    eager_att = 0
+    eager_same = 0
+    eager_wall = 0.0
+    for k in range(n_eager):
+        u = W.stage(reps[k], kind, initialize=True)
+        tic = time.perf_counter()
+        st, _ = u.run(mcs, use_graph=False)
+        x = u.getPositions(reps[k].n)
+        eager_wall += time.perf_counter() - tic
+        eager_att += st.attempted
+        eager_same += int(np.array_equal(x, b_xyz[k]))
+        u.cleanup()
     batched = attempted / batch_wall
     plain = attempted / plain_wall
+    eager = eager_att / eager_wall if eager_wall > 0 else None
     log(f"batched {mcs} MCS: {batch_ms:.1f} ms GPU, {batch_wall * 1e3:.1f} ms wall incl. read-back "
         f"-> {batched / 1e9:.2f} G attempts/s")
-    log(f"plain loop: {plain_ms:.1f} ms GPU, {plain_wall * 1e3:.1f} ms wall -> {plain / 1e9:.3f} G attempts/s; "
-        f"speedup {batched / plain:.1f}x; bit-identical {same}/{n_rep}, accepted equal {accepted_same}/{n_rep}")
+    log(f"plain loop (graphs prebuilt): {plain_ms:.1f} ms GPU, {plain_wall * 1e3:.1f} ms wall -> "
+        f"{plain / 1e9:.3f} G attempts/s; speedup {batched / plain:.1f}x; bit-identical {same}/{n_rep}, "
+        f"accepted equal {accepted_same}/{n_rep}")
+    if eager:
+        log(f"plain loop, eager launches, first {n_eager} replicas: {eager_wall * 1e3:.1f} ms wall -> "
+            f"{eager / 1e9:.3f} G attempts/s; speedup {batched / eager:.1f}x; bit-identical {eager_same}/{n_eager}")
     return {
         "workload": f"replica mix: {n_rep} systems, box edges log-uniform in {{32,64,128}} per axis, "
                     f"chains N ~ U[16,512], phi ~ U[0.1,0.5], SAW growth (ordered fallback for {n_ordered}), "
@@ -284,6 +315,12 @@ def batched_vs_plain(n_rep: int, mcs: int, seed: int, kind: int, tune_sample: in
         "plain_value": plain,
         "plain_gpu_value": attempted / (plain_ms * 1e-3),
         "speedup_vs_plain": batched / plain,
+        "plain_note": "each system's graphs built before the timed loop (prepareGraphs); runSimulationOnGPU "
+                      "per system in turn, synchronous with copy-back",
+        "plain_eager": {"value": eager, "unit": "attempted monomer moves/s (wall, incl. read-back)",
+                        "replicas": n_eager, "speedup_vs": batched / eager if eager else None,
+                        "parity": f"{eager_same}/{n_eager} bit-identical to the batch",
+                        "note": "eager launches, one host launch per kernel"},
         "batch_ms": batch_ms,
         "plain_ms_wall": plain_wall * 1e3,
         "shape": {"block": block, "slots": slots, "workgroups": used_tiles},
@@ -298,7 +335,9 @@ def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1000, help="timed MCS")
-    ap.add_argument("--warmup", type=int, default=1000, help="untimed pre-relaxation MCS")
+    ap.add_argument("--warmup", type=int, default=20, help="untimed MCS after the pre-relaxation")
+    ap.add_argument("--prerelax", type=int, default=1000,
+                    help="MCS of pre-relaxation after the ordered packing (part of building the workload)")
     ap.add_argument("--lattice", choices=["bit", "u8"], default="bit")
     ap.add_argument("--box", type=int, default=512)
     ap.add_argument("--chain", type=int, default=256)
@@ -326,6 +365,10 @@ def main() -> int:
     ap.add_argument("--network-mcs", type=int, default=2000)
     ap.add_argument("--no-network", action="store_true")
     ap.add_argument("--compare-mcs", type=int, default=300, help="MCS of the other lattice kind on the same melt")
+    ap.add_argument("--compare-cpu-seconds", type=float, default=3.0,
+                    help="target seconds of the oracle sample that checks the other lattice kind")
+    ap.add_argument("--eager-sample", type=int, default=64,
+                    help="replicas of the plain loop re-run with eager launches")
     ap.add_argument("--serve-rates", default="20,80", help="Poisson arrival rates (jobs/s) of the served stream")
     ap.add_argument("--serve-seconds", type=float, default=8.0, help="arrival window per rate")
     ap.add_argument("--no-serve", action="store_true")
@@ -354,33 +397,43 @@ def main() -> int:
     if args.graph_chunk:
         u.setGraphChunk(args.graph_chunk)
     colors = u.colors(xyz.shape[0])
+    n = int(xyz.shape[0])
     log(f"initialize done in {time.perf_counter() - tic:.1f} s; species sizes {u.colorSizes().tolist()}; "
         f"device bytes {u.deviceBytes() / 2**20:.0f} MiB")
 
+    # Pre-relaxation: part of building the workload, whatever --warmup says.
+    relaxed = None
+    if args.prerelax > 0:
+        st, _ = u.run(args.prerelax)
+        relaxed = u.getPositions(n)
+        log(f"pre-relax {args.prerelax} MCS: {st.gpu_ms:.1f} ms GPU, acceptance {st.accepted / max(st.attempted, 1):.4f}")
     if args.warmup > 0:
-        tic = time.perf_counter()
         st, _ = u.run(args.warmup)
-        log(f"pre-relax {args.warmup} MCS: {st.gpu_ms:.1f} ms GPU, acceptance {st.accepted / max(st.attempted, 1):.4f}")
+        log(f"warm-up {args.warmup} MCS: {st.gpu_ms:.1f} ms GPU, acceptance {st.accepted / max(st.attempted, 1):.4f}")
 
-    # timed region: K MCS, graph replays, synchronous at both ends
+    # timed region: K MCS of graph replays; the graphs are captured before the
+    # start event (here, and by the run itself for any it lacks)
+    u.prepareGraphs(args.steps)
     tic = time.perf_counter()
     st, _ = u.run(args.steps)
     wall = time.perf_counter() - tic
     gpu_s = st.gpu_ms / 1e3
     value = st.attempted / gpu_s
     p1, p2 = algorithmic_bytes(st, c_l)
-    log(f"timed {args.steps} MCS: {st.gpu_ms:.1f} ms GPU ({wall * 1e3:.1f} ms wall incl. copy-back), "
-        f"{value / 1e9:.2f} G attempts/s, acceptance {st.accepted / st.attempted:.4f}")
+    sweep_gbs = (p1 + p2) / gpu_s / 1e9
+    log(f"timed {args.steps} MCS: {st.gpu_ms:.2f} ms GPU ({wall * 1e3:.1f} ms wall incl. copy-back), "
+        f"{value / 1e9:.2f} G attempts/s, acceptance {st.accepted / st.attempted:.4f}, "
+        f"sweep {sweep_gbs:.0f} GB/s algorithmic ({sweep_gbs / HBM_PEAK_GBS:.3f} of peak)")
 
     # per-kernel timing: HIP events around every launch of a separate sweep
     pst, _ = u.run(args.profile_mcs, profile=True)
     k1_mean_ms = pst.check_ms / pst.check_launches
     k2_mean_ms = pst.perform_ms / pst.perform_launches
     pp1, pp2 = algorithmic_bytes(pst, c_l)
-    k1_bytes_per_launch = pp1 / pst.check_launches
-    k1_gbs = k1_bytes_per_launch / (k1_mean_ms * 1e-3) / 1e9
-    log(f"pass 1 mean {k1_mean_ms * 1e3:.1f} us ({k1_gbs:.0f} GB/s algorithmic), "
-        f"pass 2 mean {k2_mean_ms * 1e3:.1f} us")
+    k1_bytes = pp1 / pst.check_launches
+    k2_bytes = pp2 / pst.perform_launches
+    log(f"pass 1 mean {k1_mean_ms * 1e3:.1f} us ({k1_bytes / (k1_mean_ms * 1e-3) / 1e9:.0f} GB/s algorithmic), "
+        f"pass 2 mean {k2_mean_ms * 1e3:.1f} us ({k2_bytes / (k2_mean_ms * 1e-3) / 1e9:.0f} GB/s algorithmic)")
 
     # Plain streaming copy of 512 MiB: achievable-bandwidth reference, and the
     # kernel whose byte counters scale the profiler's traffic figures.
@@ -389,14 +442,39 @@ def main() -> int:
     copy_gbs = 2 * copy_bytes / (copy_ms * 1e-3) / 1e9
     log(f"copy reference: {copy_gbs:.0f} GB/s (read+write)")
 
-    # HBM bytes per pass-1 launch, from the committed rocprofv3 FETCH_SIZE /
-    # WRITE_SIZE runs of this workload (profiles/), when present.
+    # Steady-state kernel durations and HBM bytes per launch from the
+    # committed rocprofv3 runs of this workload (profiles/), when present.
+    prof = {}
+    pfile = ROOT / "profiles" / f"dense_melt_{L}_{args.lattice}.json"
+    if pfile.exists() and args.chain == 256 and abs(args.phi - 0.5) < 1e-9:
+        prof = json.loads(pfile.read_text())
+    # the profile's names of this lattice's kernels
+    names = {"k_check": "k_check" if args.lattice == "bit" else "k_check.u8",
+             "k_perform": "k_perform" if args.lattice == "bit" else "k_perform.u8"}
+    pdur = prof.get("durations", {})
+    ptraf = (prof.get("traffic") or {}).get("kernels", {})
+
+    def kernel_entry(key, bytes_per_launch, mean_ms):
+        e = {"algorithmic_bytes_per_launch": bytes_per_launch, "mean_launch_us_events": mean_ms * 1e3,
+             "achieved_GBps_events": bytes_per_launch / (mean_ms * 1e-3) / 1e9}
+        e["frac_events"] = e["achieved_GBps_events"] / HBM_PEAK_GBS
+        d = pdur.get(names[key])
+        if d:
+            e["mean_launch_us_rocprof"] = d["mean_us"]
+            e["achieved_GBps_rocprof"] = bytes_per_launch / (d["mean_us"] * 1e-6) / 1e9
+            e["frac_rocprof"] = e["achieved_GBps_rocprof"] / HBM_PEAK_GBS
+        t = ptraf.get(names[key])
+        e["traffic"] = t.get("traffic_bytes") if t else None
+        return e
+
+    kernels = {"k_check (pass 1)": kernel_entry("k_check", k1_bytes, k1_mean_ms),
+               "k_perform (pass 2)": kernel_entry("k_perform", k2_bytes, k2_mean_ms)}
+    substeps_per_mcs = st.substeps / args.steps
     traffic = None
-    tfile = ROOT / "profiles" / f"dense_melt_{L}_{args.lattice}.json"
-    if tfile.exists() and args.chain == 256 and abs(args.phi - 0.5) < 1e-9:
-        tr = json.loads(tfile.read_text()).get("traffic") or {}
-        key = "k_check" if args.lattice == "bit" else "k_check.u8"  # the profile's name of this lattice's pass 1
-        traffic = tr.get("kernels", {}).get(key, {}).get("traffic_bytes")
+    t1 = kernels["k_check (pass 1)"]["traffic"]
+    t2 = kernels["k_perform (pass 2)"]["traffic"]
+    if t1 is not None and t2 is not None:
+        traffic = t1 + t2  # one substep: one launch of each
 
     out = {
         "metric": METRIC,
@@ -412,39 +490,56 @@ def main() -> int:
         "dtype": "u32",
         "data": "synthetic",
         "config": {
-            "workload": f"dense melt: {n_chains} linear chains of N={args.chain} ({xyz.shape[0]} monomers) "
-                        f"on {L}^3 periodic, occupied-site fraction {8 * xyz.shape[0] / L**3:.3f}, "
-                        f"ordered packing + {args.warmup} MCS pre-relax, {u.nColors()} species, "
+            "workload": f"dense melt: {n_chains} linear chains of N={args.chain} ({n} monomers) "
+                        f"on {L}^3 periodic, occupied-site fraction {8 * n / L**3:.3f}, "
+                        f"ordered packing + {args.prerelax} MCS pre-relax, {u.nColors()} species, "
                         f"{args.lattice} Z-curve lattice, seed {args.seed}",
             "lattice": args.lattice,
-            "monomers": int(xyz.shape[0]),
+            "monomers": n,
             "box": L,
+            "prerelax_mcs": args.prerelax,
         },
         "roofline": {
             "bound": "hbm",
-            "achieved": k1_gbs,
+            "achieved": sweep_gbs,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
-            "frac": k1_gbs / HBM_PEAK_GBS,
+            "frac": sweep_gbs / HBM_PEAK_GBS,
             "traffic": traffic,
-            "kernel": "k_check (pass 1)",
-            "bytes_per_launch": k1_bytes_per_launch,
-            "mean_launch_us": k1_mean_ms * 1e3,
+            "kernel": "sweep: one substep = one launch of k_check (pass 1) and one of k_perform (pass 2)",
+            "algorithmic_bytes_per_substep": (p1 + p2) / st.substeps,
+            "mean_substep_us": st.gpu_ms * 1e3 / st.substeps,
+            "substeps_per_mcs": substeps_per_mcs,
+            "definition": "achieved = algorithmic bytes of the timed MCS (pass 1: n(12+9c_L)+16D per substep, "
+                          "pass 2: A(12+2c_L)) / their GPU time; traffic = rocprofv3 HBM bytes of one "
+                          "launch of each kernel",
+            "kernels": kernels,
         },
         "acceptance": st.accepted / st.attempted,
-        "sweep_algorithmic_GBps": (p1 + p2) / gpu_s / 1e9,
-        "sweep_roofline_frac": (p1 + p2) / gpu_s / 1e9 / HBM_PEAK_GBS,
-        "pass2_mean_us": k2_mean_ms * 1e3,
         "copy_reference_GBps": copy_gbs,
         "wall_ms_incl_copyback": wall * 1e3,
     }
     if args.compare_mcs > 0:
-        # the same melt on the other occupancy lattice (uint8 vs 1-bit)
+        # the same melt on the other occupancy lattice (uint8 vs 1-bit): its
+        # relaxed state must equal this one's (the lattice kind does not change
+        # a trajectory), a sample is checked against the oracle, then timed
         other = "u8" if args.lattice == "bit" else "bit"
         okind = P.LATTICE_U8 if other == "u8" else P.LATTICE_BIT
         oc_l = 1.0 if other == "u8" else 0.125
         v = make_updater(box, xyz, bonds, args.seed, okind, args.block, args.slots)
+        cross = None
+        if args.prerelax > 0:
+            v.run(args.prerelax)
+            cross = bool(np.array_equal(v.getPositions(n), relaxed))
+            log(f"{other} lattice after the {args.prerelax}-MCS pre-relax: "
+                f"{'identical to' if cross else 'DIFFERENT from'} the {args.lattice} lattice's state")
+        del relaxed
+        oparity = None
+        if not args.no_cpu and args.compare_cpu_seconds > 0:
+            _, oparity, omcs = cpu_baseline(v, box, bonds, colors, args.seed, args.compare_cpu_seconds, max_mcs=50)
+            log(f"{other} lattice vs the oracle over {omcs} MCS: {oparity}")
         v.run(min(args.warmup, 200))
+        v.prepareGraphs(args.compare_mcs)
         ost, _ = v.run(args.compare_mcs)
         v.cleanup()
         op1, op2 = algorithmic_bytes(ost, oc_l)
@@ -453,9 +548,15 @@ def main() -> int:
             "lattice": other, "mcs": args.compare_mcs, "value": ost.attempted / (ost.gpu_ms * 1e-3),
             "unit": "attempted monomer moves/s", "sweep_algorithmic_GBps": ogbs,
             "sweep_roofline_frac": ogbs / HBM_PEAK_GBS,
+            "acceptance": ost.accepted / ost.attempted,
         }
+        if cross is not None:
+            out["other_lattice"]["relaxed_state_vs_main_lattice"] = (
+                f"{'bit-identical' if cross else 'MISMATCH'} after {args.prerelax} MCS ({n} monomers)")
+        if oparity is not None:
+            out["other_lattice"]["parity_vs_oracle"] = f"{'bit-exact' if oparity else 'MISMATCH'} over {omcs} MCS"
         log(f"{other} lattice: {out['other_lattice']['value'] / 1e9:.2f} G attempts/s, "
-            f"sweep {ogbs:.0f} GB/s algorithmic ({ogbs / HBM_PEAK_GBS:.2f} of peak)")
+            f"sweep {ogbs:.0f} GB/s algorithmic ({ogbs / HBM_PEAK_GBS:.3f} of peak)")
     if not args.no_cpu:
         log("CPU baseline (oracle) on a bounded sample ...")
         cpu, parity, mcs = cpu_baseline(u, box, bonds, colors, args.seed, args.cpu_seconds)

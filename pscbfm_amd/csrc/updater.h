@@ -85,7 +85,13 @@ public:
     void setLaunchShape(uint32_t block, uint32_t slots);
     // Substeps of the bulk graph (0 = nColors * max(1, 256 / nColors)); a
     // run replays it, then power-of-two graphs cover the remainder.
-    void setGraphChunk(uint32_t substeps) { graphChunk_ = substeps; }
+    // invalid_argument above kMaxGraphChunk (each substep is two kernel nodes).
+    static constexpr uint32_t kMaxGraphChunk = 4096;
+    void setGraphChunk(uint32_t substeps);
+    // Captures and instantiates every graph a run of nMCS sweeps replays, so
+    // that the run's first launch starts at once (a run builds what is
+    // missing itself, before its timing starts).
+    void prepareGraphs(int32_t nMCS);
     // Pass-2 threads per workgroup (0 = automatic: 128 for systems at least a
     // quarter occupied, else the pass-1 block).
     void setPerformBlock(uint32_t block);
@@ -135,6 +141,11 @@ public:
     void downloadLattice(std::vector<uint8_t>& out) const;
     // Copies both proposal lattices (2 * cells / 8 bytes).
     void downloadProposal(std::vector<uint32_t>& out) const;
+    // Bonded partners in the species layout (directed) and how many of them
+    // have no 16-bit partner code (kIdEscape: the uint8-lattice kernel reads
+    // the full neighbour matrix for them).  From the initialized layout, else
+    // from the layout initialize() would build; needs no GPU.
+    void partnerCodeStats(uint64_t& partners, uint64_t& escapes) const;
 
 private:
     void requireMonomers(const char* who) const;
@@ -142,8 +153,16 @@ private:
     void freeDevice();
     void runChunk(uint64_t S, uint32_t* log, bool profile, bool useGraph);
     // Captured graph of `substeps` substeps plus their counter advance,
-    // instantiated on first use.
+    // instantiated and uploaded on first use.
     hipGraphExec_t graphFor(uint32_t substeps);
+    // The replays that cover S substeps: `bulkCount` times the bulk graph,
+    // then each graph of `tail` once (the binary digits of the remainder).
+    struct GraphPlan {
+        hipGraphExec_t bulk = nullptr;
+        uint64_t bulkCount = 0;
+        std::vector<hipGraphExec_t> tail;
+    };
+    GraphPlan planGraphs(uint64_t S);
     void dropGraphs();
     uint32_t graphChunk() const;
     void syncFromDevice();

@@ -83,6 +83,9 @@ def test_crosslinked_network_with_scattered_partners(kind):
     # neighbour matrix for them
     s = stars((128, 128, 128), 1000, 4, 29, seed=7)
     print(f"network kind {kind}: {s.n} monomers", flush=True)
+    from helpers import stage_updater
+    partners, escapes = stage_updater(s, 17, kind).partnerEscapes()
+    assert 0 < escapes < partners  # the escape path is taken
     gx, glog, ox, olog = gpu_vs_oracle(s, seed=17, n_mcs=30, kind=kind)
     assert np.array_equal(glog, olog)
     assert np.array_equal(gx, ox)

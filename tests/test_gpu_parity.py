@@ -132,12 +132,14 @@ def test_network_config_bit_exact():
     assert np.array_equal(gx, ox)
 
 
-def test_plain_config_full_length_bit_exact():
+@pytest.mark.parametrize("kind", KINDS)
+def test_plain_config_full_length_bit_exact(kind):
     # The plain single-system case in full: 64 SAW chains of 1024 on 256^3,
-    # seed 42, 10^4 MCS, compared with the oracle at the end.
+    # seed 42, 10^4 MCS, compared with the oracle at the end, on both lattice
+    # kinds (uint8 is the reference's default, UpdaterGPUScBFM_AB_Type.cu:12-13).
     s = chains((256, 256, 256), 64, 1024, seed=42)
-    print("plain full length", s.n, flush=True)
-    u = make_updater(s, 42)
+    print("plain full length", s.n, "kind", kind, flush=True)
+    u = make_updater(s, 42, kind=kind)
     u.runSimulationOnGPU(10000)
     gx = u.getPositions(s.n)
     _, acc = u.counters()
@@ -148,3 +150,47 @@ def test_plain_config_full_length_bit_exact():
     _, olog = o.run(0, 10000 * (int(colors.max()) + 1))
     assert np.array_equal(gx, o.xyz)
     assert acc == int(olog.sum())
+
+
+MELT_BOX = (512, 512, 512)
+
+
+def _melt():
+    # The dense-melt workload at full size: 32768 chains of N=256 (8,388,608
+    # monomers) on 512^3, ordered packing at occupied-site fraction 0.5.
+    return ordered_chains(MELT_BOX, 32768, 256)
+
+
+@pytest.mark.parametrize("kind", KINDS)
+def test_dense_melt_config_full_size_bit_exact(kind):
+    s = _melt()
+    print("dense melt full size", s.n, "kind", kind, flush=True)
+    gx, glog, ox, olog = gpu_vs_oracle(s, seed=42, n_mcs=3, kind=kind)
+    assert np.array_equal(glog, olog)
+    assert np.array_equal(gx, ox)
+    assert glog.sum() > 0
+
+
+def test_dense_melt_relaxed_on_both_lattices_bit_exact():
+    # 200 MCS of the full-size melt on the 1-bit and the uint8 lattice end in
+    # the same state (the lattice kind does not change a trajectory); 3 more
+    # MCS from there on the uint8 lattice match the oracle started from it.
+    from helpers import oracle_for
+    s = _melt()
+    print("dense melt relaxed", s.n, flush=True)
+    xs = {}
+    for kind in KINDS:
+        u = make_updater(s, 42, kind=kind)
+        u.runSimulationOnGPU(200)
+        xs[kind] = u.getPositions(s.n)
+        if kind == P.LATTICE_U8:
+            colors = u.colors(s.n)
+            t0, _ = u.counters()
+            _, glog = u.run(3, log=True)
+            gx = u.getPositions(s.n)
+        u.cleanup()
+    assert np.array_equal(xs[P.LATTICE_BIT], xs[P.LATTICE_U8])
+    o = oracle_for(Sys(s.box, xs[P.LATTICE_U8], s.bonds), colors, 42)
+    _, olog = o.run(t0, 3 * (int(colors.max()) + 1))
+    assert np.array_equal(glog, olog)
+    assert np.array_equal(gx, o.xyz)

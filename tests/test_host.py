@@ -455,3 +455,51 @@ def test_cli_seed_file_prints_the_reference_known_answers(tmp_path):
     assert "RNG rand: 4223731124" in r.stderr and "RNG rand: 0.803876" in r.stderr
     # without a GPU the run stops at device selection; with one it completes
     assert r.returncode == 0 or "setGpu" in r.stderr
+
+
+# ---- run setup without a GPU -------------------------------------------------------
+def test_graph_chunk_is_bounded():
+    u = P.UpdaterGPUScBFM_AB_Type()
+    u.setGraphChunk(4096)
+    u.setGraphChunk(0)
+    for bad in (4097, 2**31, 2**32 - 1):
+        with pytest.raises(P.InvalidArgument, match="4096"):
+            u.setGraphChunk(bad)
+
+
+def test_prepare_graphs_before_initialize_is_runtime_error():
+    u = _filled(1)
+    with pytest.raises(RuntimeError, match="initialize"):
+        u.prepareGraphs(10)
+
+
+def _staged_without_gpu(s: Sys):
+    from pscbfm_amd import workloads as W
+    sd = W.SystemData(s.box, s.xyz, s.bonds, 1)
+    u = P.UpdaterGPUScBFM_AB_Type()
+    u.setPeriodicity(True, True, True)
+    u.setNrOfAllMonomers(sd.n)
+    u.setMonomerCoordinatesBulk(sd.xyz)
+    u.setConnectivityBulk(W.directed_pairs(sd.n, sd.bonds))
+    u.setLatticeSize(*sd.box)
+    forb = W.standard_forbidden()
+    for dx, dy, dz in itertools.product(range(-4, 4), repeat=3):
+        u.copyBondSet(dx, dy, dz, bool(forb[(dx & 7) | ((dy & 7) << 3) | ((dz & 7) << 6)]))
+    return u
+
+
+def test_partner_codes_of_chains_need_no_escape():
+    # ordered and grown chains keep every partner near its own rank in the
+    # partner's species, so all partners have a 16-bit code
+    for s in (ordered_chains((64, 64, 64), 256, 32), chains((64, 64, 64), 40, 64, seed=3)):
+        partners, escapes = _staged_without_gpu(s).partnerEscapes()
+        assert partners == 2 * s.bonds.shape[0] and escapes == 0
+
+
+def test_crosslinked_network_has_partner_escapes():
+    # the network of the GPU test of scattered partners: crosslinks join
+    # monomers whose slots lie far apart, which the 16-bit codes cannot span
+    s = stars((128, 128, 128), 1000, 4, 29, seed=7)
+    partners, escapes = _staged_without_gpu(s).partnerEscapes()
+    assert partners == 2 * s.bonds.shape[0]
+    assert 0 < escapes < partners
