@@ -347,6 +347,9 @@ def main() -> int:
     ap.add_argument("--slots", type=int, default=0, help="monomers per pass-1 thread (0: default)")
     ap.add_argument("--perform-block", type=int, default=0, help="pass-2 threads per workgroup (0: default)")
     ap.add_argument("--graph-chunk", type=int, default=0, help="substeps per captured graph (0: default)")
+    ap.add_argument("--list-group", type=int, default=0,
+                    help="pass-2 threads per pass-1 list (0: default; -1: one list per workgroup)")
+    ap.add_argument("--list-depth", type=int, default=0, help="pass-2 entries per thread in flight (0: default)")
     ap.add_argument("--profile-mcs", type=int, default=20)
     ap.add_argument("--cpu-seconds", type=float, default=20.0,
                     help="target seconds of the all-cores oracle sample (its calibration run overestimates, "
@@ -396,6 +399,8 @@ def main() -> int:
         u.setPerformBlock(args.perform_block)
     if args.graph_chunk:
         u.setGraphChunk(args.graph_chunk)
+    if args.list_group or args.list_depth:
+        u.setPerformWalk(P.LEGACY_WALK if args.list_group < 0 else args.list_group, args.list_depth)
     colors = u.colors(xyz.shape[0])
     n = int(xyz.shape[0])
     log(f"initialize done in {time.perf_counter() - tic:.1f} s; species sizes {u.colorSizes().tolist()}; "

@@ -138,6 +138,12 @@ int pscbfm_set_launch_shape(pscbfm_updater* h, uint32_t block, uint32_t slots);
  * 128 when at least a quarter of the cells are occupied, else the pass-1
  * block).  Results do not depend on it. */
 int pscbfm_set_perform_block(pscbfm_updater* h, uint32_t block);
+/* Pass-2 walk of a single system: threads_per_list threads walk one pass-1
+ * list (64..256, multiple of 64), so a pass-2 workgroup walks several lists,
+ * with entries_per_thread entries in flight per thread (1 or 4; 0 = 4);
+ * threads_per_list = 0 or 0xFFFFFFFF (the default) walks one list per
+ * workgroup.  Results do not depend on it. */
+int pscbfm_set_perform_walk(pscbfm_updater* h, uint32_t threads_per_list, uint32_t entries_per_thread);
 /* Substeps per captured graph (0 = automatic, at most 4096:
  * PSCBFM_INVALID_ARGUMENT above). */
 int pscbfm_set_graph_chunk(pscbfm_updater* h, uint32_t substeps);

@@ -30,6 +30,7 @@ __all__ = [
 
 LATTICE_U8 = 0
 LATTICE_BIT = 1
+LEGACY_WALK = 0xFFFFFFFF  # setPerformWalk: one pass-1 list per pass-2 workgroup
 
 _HERE = Path(__file__).resolve().parent
 library_path = _HERE / "libpscbfm.so"
@@ -133,6 +134,7 @@ _SIGNATURES = {
     "pscbfm_set_launch_shape": (C.c_int, [_H, C.c_uint32, C.c_uint32]),
     "pscbfm_set_perform_block": (C.c_int, [_H, C.c_uint32]),
     "pscbfm_set_graph_chunk": (C.c_int, [_H, C.c_uint32]),
+    "pscbfm_set_perform_walk": (C.c_int, [_H, C.c_uint32, C.c_uint32]),
     "pscbfm_prepare_graphs": (C.c_int, [_H, C.c_int32]),
     "pscbfm_run_ex": (C.c_int, [_H, C.c_int32, _u32p, C.c_int, C.c_int, C.POINTER(RunStats)]),
     "pscbfm_last_run_stats": (C.c_int, [_H, C.POINTER(RunStats)]),
@@ -364,6 +366,11 @@ class UpdaterGPUScBFM_AB_Type:
         """Pass-2 threads per workgroup (0: automatic, 128 for systems at least a
         quarter occupied, else the pass-1 block)."""
         self._call("pscbfm_set_perform_block", int(block))
+
+    def setPerformWalk(self, threads_per_list: int = 0, entries_per_thread: int = 0) -> None:
+        """Pass-2 walk: threads per pass-1 list (0 or LEGACY_WALK, the default:
+        one list per workgroup) and entries in flight per thread (1 or 4; 0: 4)."""
+        self._call("pscbfm_set_perform_walk", int(threads_per_list), int(entries_per_thread))
 
     def setGraphChunk(self, substeps: int) -> None:
         self._call("pscbfm_set_graph_chunk", int(substeps))

@@ -286,6 +286,10 @@ int pscbfm_set_launch_shape(pscbfm_updater* h, uint32_t block, uint32_t slots) {
     return guarded(h, [&](pscbfm::Updater& u) { u.setLaunchShape(block, slots); });
 }
 
+int pscbfm_set_perform_walk(pscbfm_updater* h, uint32_t threads_per_list, uint32_t entries_per_thread) {
+    return guarded(h, [&](pscbfm::Updater& u) { u.setPerformWalk(threads_per_list, entries_per_thread); });
+}
+
 int pscbfm_set_graph_chunk(pscbfm_updater* h, uint32_t substeps) {
     return guarded(h, [&](pscbfm::Updater& u) { u.setGraphChunk(substeps); });
 }

@@ -17,11 +17,40 @@
 
 #if defined(__HIPCC__)
 #define PSCBFM_HD __host__ __device__ inline
+#define PSCBFM_HOST_ONLY __host__ inline
 #else
 #define PSCBFM_HD inline
+#define PSCBFM_HOST_ONLY inline
 #endif
 
 namespace pscbfm {
+
+#if defined(__HIPCC__)
+// A pointer into global memory, as the ISA's GLOBAL instructions address it.
+template <class T>
+using gptr = __attribute__((address_space(1))) T*;
+#endif
+
+// A device buffer as a descriptor carries it.  Host code stores and reads the
+// address (implicit conversions, host only); device code can reach the
+// buffer only through global(), a global-address-space pointer.  A pointer
+// read from a descriptor in memory (a batch's descriptor table) would
+// otherwise be generic, and the compiler would address it with FLAT
+// instructions, whose completion waits on gfx950 proved insufficient for the
+// sweep's gathers under load.
+template <class T>
+class DeviceBuffer {
+public:
+    DeviceBuffer() = default;
+    PSCBFM_HOST_ONLY DeviceBuffer(T* p) : p_(p) {}
+    PSCBFM_HOST_ONLY operator T*() const { return p_; }
+#if defined(__HIPCC__)
+    __device__ inline gptr<T> global() const { return (gptr<T>)p_; }
+#endif
+
+private:
+    T* p_ = nullptr;
+};
 
 // Upper bound of bonds per monomer.  The reference compiles with 4 and notes
 // that the hydrogel coloring example needs 8 (UpdaterGPUScBFM_AB_Type.h:38-39);
@@ -225,20 +254,20 @@ struct Tile {
 
 // Everything a sweep kernel needs to know about one system.
 struct SysDesc {
-    uint32_t* pos;           // Z-curve code per slot
-    int32_t* imgX;           // periodic image counters per slot
-    int32_t* imgY;
-    int32_t* imgZ;
-    const uint32_t* nbr;     // neighbour matrices (slot ids)
-    void* lattice;           // uint8 per cell or 1 bit per cell
-    uint32_t* proposal;      // two 1-bit proposal lattices, latWords each
-    const ColorInfo* colors; // nColors entries
-    const uint32_t* bondMask;// 16 words: bit k set <=> Z-order bond key k forbidden
-    DevState* state;
-    uint2* list;             // pass-1 survivors, one list per pass-1 workgroup
-    uint32_t* listCount;     // entries of each list
-    const uint32_t* ids;     // compressed partner ids (see layout.h)
-    const uint32_t* rowBase; // kMaxConnectivity per species: base slot of each row's deltas
+    DeviceBuffer<uint32_t> pos;           // Z-curve code per slot
+    DeviceBuffer<int32_t> imgX;           // periodic image counters per slot
+    DeviceBuffer<int32_t> imgY;
+    DeviceBuffer<int32_t> imgZ;
+    DeviceBuffer<const uint32_t> nbr;     // neighbour matrices (slot ids)
+    DeviceBuffer<uint8_t> lattice;        // uint8 per cell or 1 bit per cell
+    DeviceBuffer<uint32_t> proposal;      // two 1-bit proposal lattices, latWords each
+    DeviceBuffer<const ColorInfo> colors; // nColors entries
+    DeviceBuffer<const uint32_t> bondMask;// 16 words: bit k set <=> Z-order bond key k forbidden
+    DeviceBuffer<DevState> state;
+    DeviceBuffer<uint2> list;             // pass-1 survivors, one list per pass-1 workgroup
+    DeviceBuffer<uint32_t> listCount;     // entries of each list
+    DeviceBuffer<const uint32_t> ids;     // compressed partner ids (see layout.h)
+    DeviceBuffer<const uint32_t> rowBase; // kMaxConnectivity per species: base slot of each row's deltas
     uint64_t seed;           // key of the substep draw
     uint32_t mask[3];        // Z-curve bit masks of x, y, z
     uint32_t nColors;

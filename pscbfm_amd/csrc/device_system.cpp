@@ -14,6 +14,8 @@
 //   acc      uint64 per workgroup accepted-move running totals
 //   ids      16-bit partner codes, two per word (uint8 lattice only)
 //   rowBase  uint32 per (species, row): base slot of the codes (uint8 lattice only)
+//   slotOf   uint32 per monomer   slot of each monomer (only with withLists)
+//   xyz      3 int32 per monomer  unfolded coordinates decoded for read-back (only with withLists)
 #include "device_system.h"
 
 #include <algorithm>
@@ -37,7 +39,8 @@ namespace {
 
 uint64_t align_up(uint64_t v, uint64_t a) { return (v + a - 1) / a * a; }
 
-enum { P_POS, P_IMGX, P_IMGY, P_IMGZ, P_LIST, P_LCOUNT, P_NBR, P_LAT, P_PROP, P_COLORS, P_BOND, P_STATE, P_ACC, P_IDS, P_ROWB, P_N };
+enum { P_POS, P_IMGX, P_IMGY, P_IMGZ, P_LIST, P_LCOUNT, P_NBR, P_LAT, P_PROP, P_COLORS, P_BOND, P_STATE, P_ACC, P_IDS, P_ROWB,
+       P_SLOTOF, P_XYZ, P_N };
 
 struct Plan {
     uint64_t bytes[P_N];
@@ -67,6 +70,10 @@ Plan plan(const SystemLayout& L, uint32_t latticeKind, bool withLists, uint32_t 
     const bool codes = latticeKind == kLatticeU8;
     p.bytes[P_IDS] = codes ? std::max<uint64_t>(L.ids.size(), 1) * 4 : 4;
     p.bytes[P_ROWB] = codes ? std::max<uint64_t>(L.rowBase.size(), 1) * 4 : 4;
+    // a system with its own launches decodes its coordinates on the device
+    const uint64_t nMono = std::max<uint64_t>(L.nMonomers, 1);
+    p.bytes[P_SLOTOF] = withLists ? nMono * 4 : 4;
+    p.bytes[P_XYZ] = withLists ? nMono * 12 : 4;
     p.total = 0;
     for (int i = 0; i < P_N; ++i) {
         p.off[i] = p.total;
@@ -100,7 +107,7 @@ void DeviceSystem::allocate(const SystemLayout& L, uint32_t latticeKind, uint64_
     desc.list = reinterpret_cast<uint2*>(base + p.off[P_LIST]);
     desc.listCount = reinterpret_cast<uint32_t*>(base + p.off[P_LCOUNT]);
     desc.nbr = reinterpret_cast<const uint32_t*>(base + p.off[P_NBR]);
-    desc.lattice = base + p.off[P_LAT];
+    desc.lattice = reinterpret_cast<uint8_t*>(base + p.off[P_LAT]);
     desc.proposal = reinterpret_cast<uint32_t*>(base + p.off[P_PROP]);
     desc.colors = reinterpret_cast<const ColorInfo*>(base + p.off[P_COLORS]);
     desc.bondMask = reinterpret_cast<const uint32_t*>(base + p.off[P_BOND]);
@@ -117,12 +124,15 @@ void DeviceSystem::allocate(const SystemLayout& L, uint32_t latticeKind, uint64_
     desc.latticeKind = latticeKind;
     desc.maxColorCount = L.maxColorCount;
     desc.accEntries = this->accEntries;
+    slotOfDev = withLists ? reinterpret_cast<uint32_t*>(base + p.off[P_SLOTOF]) : nullptr;
+    xyzDev = withLists ? reinterpret_cast<int32_t*>(base + p.off[P_XYZ]) : nullptr;
 
     hip_check(hipMemsetAsync(arena, 0, p.total, stream), "hipMemsetAsync");
     auto up = [&](int piece, const void* src, uint64_t n) {
         if (n) hip_check(hipMemcpyAsync(base + p.off[piece], src, n, hipMemcpyHostToDevice, stream), "H2D");
     };
     up(P_POS, L.pos.data(), uint64_t(L.nSlots) * 4);
+    if (withLists) up(P_SLOTOF, L.slotOf.data(), uint64_t(L.nMonomers) * 4);
     up(P_IMGX, L.img[0].data(), uint64_t(L.nSlots) * 4);
     up(P_IMGY, L.img[1].data(), uint64_t(L.nSlots) * 4);
     up(P_IMGZ, L.img[2].data(), uint64_t(L.nSlots) * 4);
@@ -161,6 +171,8 @@ void DeviceSystem::release() {
     arena = nullptr;
     bytes = 0;
     desc = SysDesc{};
+    slotOfDev = nullptr;
+    xyzDev = nullptr;
 }
 
 DevState DeviceSystem::readState(hipStream_t stream) const {
@@ -190,9 +202,19 @@ uint64_t DeviceSystem::acceptedTotal(uint32_t entries, hipStream_t stream) const
 void DeviceSystem::positions(const SystemLayout& L, int32_t* xyzOut, hipStream_t stream) const {
     const uint32_t ns = L.nSlots;
     if (!ns) return;
+    if (xyzDev) {
+        // decoded on the device in the caller's order: 12 bytes per monomer
+        // cross the bus and the host only receives them
+        if (!L.nMonomers) return;
+        hip_check(launch_unfold(desc, slotOfDev, L.nMonomers, xyzDev, stream), "launch unfold");
+        hip_check(hipMemcpyAsync(xyzOut, xyzDev, uint64_t(L.nMonomers) * 12, hipMemcpyDeviceToHost, stream),
+                  "positions D2H");
+        hip_check(hipStreamSynchronize(stream), "sync");
+        return;
+    }
     // pos, imgX, imgY and imgZ are consecutive pieces of the arena: one copy
-    const char* first = reinterpret_cast<const char*>(desc.pos);
-    const uint64_t span = uint64_t(reinterpret_cast<const char*>(desc.imgZ) - first) + uint64_t(ns) * 4;
+    const char* first = reinterpret_cast<const char*>(static_cast<uint32_t*>(desc.pos));
+    const uint64_t span = uint64_t(reinterpret_cast<const char*>(static_cast<int32_t*>(desc.imgZ)) - first) + uint64_t(ns) * 4;
     // Staging: a per-thread buffer reused across calls up to 64 MiB (no page
     // faults inside the copy for the many small systems of a batch); larger
     // systems take a one-off buffer without zero fill.
@@ -213,7 +235,7 @@ void DeviceSystem::positions(const SystemLayout& L, int32_t* xyzOut, hipStream_t
 }
 
 void DeviceSystem::decodePositions(const SystemLayout& L, const void* copy, int32_t* xyzOut) const {
-    const char* first = reinterpret_cast<const char*>(desc.pos);
+    const char* first = reinterpret_cast<const char*>(static_cast<uint32_t*>(desc.pos));
     const char* h = static_cast<const char*>(copy);
     auto at = [&](const void* dev) { return h + (reinterpret_cast<const char*>(dev) - first); };
     const int32_t* img[3] = {reinterpret_cast<const int32_t*>(at(desc.imgX)),

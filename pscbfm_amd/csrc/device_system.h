@@ -24,6 +24,11 @@ struct DeviceSystem {
     uint64_t cells = 0;
     uint32_t accEntries = 0;  // entries of DevState::acceptedByBlock
     SysDesc desc{};
+    // Systems with their own launches (withLists): the slot of every monomer
+    // and a buffer of unfolded coordinates in monomer order, so positions()
+    // decodes on the device.
+    uint32_t* slotOfDev = nullptr;
+    int32_t* xyzDev = nullptr;
 
     // Bytes allocate() would take.  `withLists`: the system runs its own
     // launches (pass-1 lists inside the allocation); batch replicas use the
@@ -43,9 +48,9 @@ struct DeviceSystem {
     // The pieces positions() copies: pos, imgX, imgY and imgZ are consecutive
     // in the arena, so one copy of positionsBytes() bytes from positionsBase()
     // holds them; decodePositions() turns such a copy into coordinates.
-    const void* positionsBase() const { return desc.pos; }
+    const void* positionsBase() const { return static_cast<uint32_t*>(desc.pos); }
     uint64_t positionsBytes(const SystemLayout& L) const {
-        return uint64_t(reinterpret_cast<const char*>(desc.imgZ) - reinterpret_cast<const char*>(desc.pos)) +
+        return uint64_t(reinterpret_cast<const char*>(static_cast<int32_t*>(desc.imgZ)) - reinterpret_cast<const char*>(static_cast<uint32_t*>(desc.pos))) +
                uint64_t(L.nSlots) * 4;
     }
     void decodePositions(const SystemLayout& L, const void* copy, int32_t* xyzOut) const;

@@ -13,10 +13,24 @@ namespace pscbfm {
 struct LaunchShape {
     uint32_t block = 256;        // pass-1 threads per workgroup (64..256, multiple of 64)
     uint32_t slots = 2;          // monomers per pass-1 thread (1, 2 or 4)
-    uint32_t grid = 8;           // workgroups (multiple of 8)
+    uint32_t grid = 8;           // pass-1 workgroups (multiple of 8)
     uint32_t performBlock = 0;   // pass-2 threads per workgroup (0: the pass-1 block)
+    // Pass 2 of a single system: 0 = one pass-1 list per workgroup; else
+    // each group of listGroup threads (a multiple of 64) walks one list, so
+    // a workgroup walks passTwoBlock() / listGroup lists, with listDepth
+    // entries (1 or 4) in flight per thread.
+    uint32_t listGroup = 0;
+    uint32_t listDepth = 4;
     uint32_t passTwoBlock() const { return performBlock ? performBlock : block; }
     uint32_t listStride() const { return slots * block; }  // list entries per workgroup
+    // Pass-1 lists each XCD produces: workgroup w runs on XCD w % 8 and
+    // writes list (w % 8) * grid / 8 + w / 8.
+    uint32_t listsPerXcd() const { return grid / 8; }
+    uint32_t passTwoGrid() const {
+        if (!listGroup) return grid;
+        const uint32_t perGroup = passTwoBlock() / listGroup;
+        return 8 * ((listsPerXcd() + perGroup - 1) / perGroup);
+    }
 };
 
 // One substep = pass 1 (propose/check) then pass 2 (resolve/apply).  `k` is
@@ -45,6 +59,9 @@ hipError_t launch_advance_batch(const SysDesc* descs, uint32_t nDescs, uint32_t 
 hipError_t launch_set_stops(const SysDesc* descs, uint32_t nDescs, const uint64_t* stops, hipStream_t st);
 // out[3r..3r+2] = substep, stopAt, accepted total of replica r.
 hipError_t launch_gather_state(const SysDesc* descs, uint32_t nDescs, uint64_t* out, hipStream_t st);
+// xyz[3i..3i+2] = unfolded corner of monomer i (slot slotOf[i]): the
+// Z-code's axis bits plus image counter times box edge, for i < n.
+hipError_t launch_unfold(const SysDesc& d, const uint32_t* slotOf, uint32_t n, int32_t* xyz, hipStream_t st);
 // Fills n 32-bit words with v (graph-safe replacement of a memset).
 hipError_t launch_fill_u32(uint32_t* p, uint64_t n, uint32_t v, hipStream_t st);
 // Streaming copy of n16 16-byte elements (bandwidth and counter reference).

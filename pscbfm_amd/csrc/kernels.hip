@@ -53,8 +53,6 @@ __device__ __forceinline__ uint32_t xcd_block() {
 // instructions, which the single-system kernels always get, are tracked by
 // the vector-memory counter alone.
 template <class T>
-using gptr = __attribute__((address_space(1))) T*;
-template <class T>
 __device__ __forceinline__ gptr<T> global_ptr(T* p) {
     return (gptr<T>)p;
 }
@@ -69,9 +67,9 @@ typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 struct Launch {
     uint64_t t;
     SubstepDraw draw;
-    uint32_t* log;
+    gptr<uint32_t> log;
     uint64_t logFirst, logCapacity;
-    uint64_t* acceptedByBlock;
+    gptr<uint64_t> acceptedByBlock;
 };
 
 __device__ __forceinline__ uint64_t join64(uint32_t lo, uint32_t hi) { return uint64_t(lo) | (uint64_t(hi) << 32); }
@@ -80,7 +78,7 @@ __device__ __forceinline__ uint64_t join64(uint32_t lo, uint32_t hi) { return ui
 // draw is entry t - drawBase of the explicit table when the state holds one,
 // else the counter draw of (seed, t).
 __device__ __forceinline__ bool begin_substep(const SysDesc& d, uint32_t k, Launch& out) {
-    const gptr<const u32x4> w = (gptr<const u32x4>)global_ptr(d.state);
+    const gptr<const u32x4> w = (gptr<const u32x4>)d.state.global();
     const u32x4 w0 = w[0], w1 = w[1], w2 = w[2], w3 = w[3];  // DevState field order
     const uint64_t t = join64(w0.x, w0.y) + k;
     // A state whose draw table, log or accepted counts are not aligned to
@@ -98,16 +96,16 @@ __device__ __forceinline__ bool begin_substep(const SysDesc& d, uint32_t k, Laun
         out.draw.species = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(e));
         out.draw.seed = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(e >> 32));
     }
-    out.log = reinterpret_cast<uint32_t*>(uintptr_t(join64(w2.x, w2.y)));
+    out.log = (gptr<uint32_t>)(uintptr_t(join64(w2.x, w2.y)));
     out.logFirst = join64(w2.z, w2.w);
     out.logCapacity = join64(w3.x, w3.y);
-    out.acceptedByBlock = reinterpret_cast<uint64_t*>(uintptr_t(join64(w3.z, w3.w)));
+    out.acceptedByBlock = (gptr<uint64_t>)(uintptr_t(join64(w3.z, w3.w)));
     return true;
 }
 
 // Species record, read word by word through a global pointer.
-__device__ __forceinline__ ColorInfo load_color(const ColorInfo* colors, uint32_t species) {
-    const gptr<const uint32_t> w = (gptr<const uint32_t>)global_ptr(colors + species);
+__device__ __forceinline__ ColorInfo load_color(gptr<const ColorInfo> colors, uint32_t species) {
+    const gptr<const uint32_t> w = (gptr<const uint32_t>)(colors + species);
     ColorInfo c{};
     c.slotOffset = w[offsetof(ColorInfo, slotOffset) / 4];
     c.count = w[offsetof(ColorInfo, count) / 4];
@@ -212,27 +210,27 @@ template <int LAT, int SLOTS, int MAXDEG>
 __device__ __forceinline__ void check_body(const SysDesc& d, uint64_t t, SubstepDraw draw, uint32_t lb,
                                            gptr<uint64_t> list, gptr<uint32_t> count, uint32_t* sForbidden,
                                            uint32_t* sCount) {
-    const ColorInfo ci = load_color(d.colors, draw.species);
-    const gptr<uint32_t> proposal = global_ptr(d.proposal);
+    const ColorInfo ci = load_color(d.colors.global(), draw.species);
+    const gptr<uint32_t> proposal = d.proposal.global();
 
-    if (threadIdx.x < 16) sForbidden[threadIdx.x] = global_ptr(d.bondMask)[threadIdx.x];
+    if (threadIdx.x < 16) sForbidden[threadIdx.x] = d.bondMask.global()[threadIdx.x];
     if (threadIdx.x == 0) *sCount = 0;
     __syncthreads();
 
     const uint32_t B = blockDim.x;
     const uint32_t base = lb * (SLOTS * B);
-    const gptr<const uint32_t> posAll = global_ptr(static_cast<const uint32_t*>(d.pos));
+    const gptr<const uint32_t> posAll = (gptr<const uint32_t>)d.pos.global();
     const gptr<const uint32_t> pos = posAll + ci.slotOffset;
-    const gptr<const uint32_t> nbr = global_ptr(d.nbr);
+    const gptr<const uint32_t> nbr = d.nbr.global();
     // Partner ids: the bit-lattice kernel reads the full matrix rows; the
     // uint8-lattice kernel, whose plane words cost more traffic, reads the
     // 16-bit codes (half the bytes) and the full matrix only for escapes.
-    const gptr<const uint32_t> ids = global_ptr(d.ids) + ci.idsOffset;
+    const gptr<const uint32_t> ids = d.ids.global() + ci.idsOffset;
     uint32_t rb[MAXDEG];
 #pragma unroll
     for (int r = 0; r < MAXDEG; ++r)
-        rb[r] = LAT == kLatticeU8 ? global_ptr(d.rowBase)[draw.species * kMaxConnectivity + uint32_t(r)] : 0u;
-    const gptr<const uint64_t> lattice = global_ptr(static_cast<const uint64_t*>(d.lattice));
+        rb[r] = LAT == kLatticeU8 ? d.rowBase.global()[draw.species * kMaxConnectivity + uint32_t(r)] : 0u;
+    const gptr<const uint64_t> lattice = (gptr<const uint64_t>)d.lattice.global();
     const AxisMasks masks = axis_masks(d);
 
     // The load phase is branch-free: lanes without a monomer (or without a
@@ -330,6 +328,50 @@ __device__ __forceinline__ uint32_t block_sum(uint32_t v, uint32_t* scratch) {
     return total;
 }
 
+// Applies an accepted move of the monomer in `slot` from code m in
+// direction dir (pass 2, CRITICAL.cu:400-408): its position, the occupancy
+// of the old and new corner, and the periodic image counter on a wrap.
+template <int LAT>
+__device__ __forceinline__ void apply_move(const SysDesc& d, gptr<uint32_t> pos, gptr<uint8_t> lat, uint32_t slot,
+                                           uint32_t m, uint32_t dir, const Move& mv) {
+    pos[slot] = mv.to;
+    if (LAT == kLatticeU8) {
+        lat[m] = 0;
+        lat[mv.to] = 1;
+    } else {
+        // Byte b of the bit lattice holds the aligned 2x2x2 block of cells
+        // 8b..8b+7 (the low three code bits are the lowest bit of x, y and
+        // z).  Two corners in one block would overlap, and an accepted move's
+        // old and new corner cannot share a block with any other monomer's
+        // old or new corner, so every byte touched here has exactly one
+        // writer and its new value is known: 0 for the vacated block, the
+        // single new bit for the entered one.
+        const uint32_t bFrom = m >> 3, bTo = mv.to >> 3;
+        if (bFrom != bTo) lat[bFrom] = 0;
+        lat[bTo] = uint8_t(1u << (mv.to & 7u));
+    }
+    if (mv.wraps) {
+        const uint32_t axis = dir >> 1;
+        const gptr<int32_t> img = axis == 0 ? d.imgX.global() : (axis == 1 ? d.imgY.global() : d.imgZ.global());
+        img[slot] += (dir & 1u) ? 1 : -1;
+    }
+}
+
+// Zeroes slice lb of nb of the proposal buffer of the other parity, 16 bytes
+// per thread and step: pass 2 of substep t-1 was its last reader and pass 1
+// of substep t+1 is its next writer.
+__device__ __forceinline__ void clear_other_parity(const SysDesc& d, uint64_t t, uint32_t lb, uint32_t nb) {
+    const gptr<uint64_t> prev = (gptr<uint64_t>)(d.proposal.global() + size_t((t + 1u) & 1u) * d.latWords);
+    const uint32_t quads = d.latWords >> 2;
+    const uint32_t per = (quads + nb - 1) / nb;
+    const uint32_t q0 = lb * per;
+    const uint32_t q1 = min(q0 + per, quads);
+    for (uint32_t q = q0 + threadIdx.x; q < q1; q += blockDim.x) {
+        prev[2 * size_t(q)] = 0u;
+        prev[2 * size_t(q) + 1] = 0u;
+    }
+}
+
 // Pass 2 of substep t over one pass-1 list.  The accepted count is added to
 // *accSlot, an entry only this workgroup writes.
 template <int LAT>
@@ -338,30 +380,17 @@ __device__ __forceinline__ void perform_body(const SysDesc& d, const Launch& L, 
                                              uint32_t* sSum) {
     const uint64_t t = L.t;
     const SubstepDraw draw = L.draw;
-    const ColorInfo ci = load_color(d.colors, draw.species);
+    const ColorInfo ci = load_color(d.colors.global(), draw.species);
     const uint32_t B = blockDim.x;
 
-    // Zero the proposal buffer of the other parity, one slice per workgroup
-    // (16 bytes per thread and step): pass 2 of substep t-1 was its last
-    // reader and pass 1 of substep t+1 its next writer.  Streaming these
-    // stores here overlaps them with the latency-bound list walk.
-    {
-        const gptr<uint64_t> prev =
-            (gptr<uint64_t>)(global_ptr(d.proposal) + size_t((t + 1u) & 1u) * d.latWords);
-        const uint32_t quads = d.latWords >> 2;
-        const uint32_t per = (quads + nb - 1) / nb;
-        const uint32_t q0 = lb * per;
-        const uint32_t q1 = min(q0 + per, quads);
-        for (uint32_t q = q0 + threadIdx.x; q < q1; q += B) {
-            prev[2 * size_t(q)] = 0u;
-            prev[2 * size_t(q) + 1] = 0u;
-        }
-    }
+    // The other parity's proposal clear, one slice per workgroup: streaming
+    // these stores here overlaps them with the latency-bound list walk.
+    clear_other_parity(d, t, lb, nb);
     const AxisMasks masks = axis_masks(d);
     const gptr<const uint64_t> T64 =
-        (gptr<const uint64_t>)(global_ptr(static_cast<const uint32_t*>(d.proposal)) + size_t(t & 1u) * d.latWords);
-    const gptr<uint32_t> pos = global_ptr(d.pos);
-    const gptr<uint8_t> lat = global_ptr(static_cast<uint8_t*>(d.lattice));
+        (gptr<const uint64_t>)(d.proposal.global() + size_t(t & 1u) * d.latWords);
+    const gptr<uint32_t> pos = d.pos.global();
+    const gptr<uint8_t> lat = d.lattice.global();
 
     // The list is compacted, so about a fifth of its capacity is filled: walk
     // only the entries that exist, one per thread and round.
@@ -373,28 +402,7 @@ __device__ __forceinline__ void perform_body(const SysDesc& d, const Launch& L, 
         const uint32_t m = uint32_t(e >> 32);
         const Move mv = make_move(m, dir, masks);
         if (plane_any<6>(T64, mv)) continue;
-        const uint32_t slot = ci.slotOffset + jj;
-        pos[slot] = mv.to;
-        if (LAT == kLatticeU8) {
-            lat[m] = 0;
-            lat[mv.to] = 1;
-        } else {
-            // Byte b of the bit lattice holds the aligned 2x2x2 block of
-            // cells 8b..8b+7 (the low three code bits are the lowest bit of
-            // x, y and z).  Two corners in one block would overlap, and an
-            // accepted move's old and new corner cannot share a block with
-            // any other monomer's old or new corner, so every byte touched
-            // here has exactly one writer and its new value is known:
-            // 0 for the vacated block, the single new bit for the entered one.
-            const uint32_t bFrom = m >> 3, bTo = mv.to >> 3;
-            if (bFrom != bTo) lat[bFrom] = 0;
-            lat[bTo] = uint8_t(1u << (mv.to & 7u));
-        }
-        if (mv.wraps) {
-            const uint32_t axis = dir >> 1;
-            const gptr<int32_t> img = global_ptr(axis == 0 ? d.imgX : (axis == 1 ? d.imgY : d.imgZ));
-            img[slot] += (dir & 1u) ? 1 : -1;
-        }
+        apply_move<LAT>(d, pos, lat, ci.slotOffset + jj, m, dir, mv);
         ++accepted;
     }
     const uint32_t total = block_sum(accepted, sSum);
@@ -403,7 +411,7 @@ __device__ __forceinline__ void perform_body(const SysDesc& d, const Launch& L, 
         if (L.log != nullptr) {
             const uint64_t idx = t - L.logFirst;
             if (idx < L.logCapacity)
-                __hip_atomic_fetch_add(&global_ptr(L.log)[idx], total, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                __hip_atomic_fetch_add(&L.log[idx], total, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
     }
 }
@@ -417,8 +425,8 @@ __global__ __launch_bounds__(256) void k_check(SysDesc d, uint32_t k) {
     if (!begin_substep(d, k, L)) return;
     const uint32_t lb = xcd_block();
     check_body<LAT, SLOTS, MAXDEG>(d, L.t, L.draw, lb,
-                                   global_ptr(reinterpret_cast<uint64_t*>(d.list)) + size_t(lb) * (SLOTS * blockDim.x),
-                                   global_ptr(d.listCount) + lb, sForbidden, &sCount);
+                                   (gptr<uint64_t>)d.list.global() + size_t(lb) * (SLOTS * blockDim.x),
+                                   d.listCount.global() + lb, sForbidden, &sCount);
 }
 
 template <int LAT>
@@ -427,8 +435,80 @@ __global__ __launch_bounds__(256) void k_perform(SysDesc d, uint32_t k, uint32_t
     Launch L;
     if (!begin_substep(d, k, L)) return;
     const uint32_t lb = xcd_block();  // same numbering as pass 1: list lb
-    perform_body<LAT>(d, L, lb, gridDim.x, global_ptr(reinterpret_cast<const uint64_t*>(d.list)) + size_t(lb) * listStride,
-                      global_ptr(d.listCount)[lb], global_ptr(L.acceptedByBlock) + blockIdx.x, sSum);
+    perform_body<LAT>(d, L, lb, gridDim.x, (gptr<const uint64_t>)d.list.global() + size_t(lb) * listStride,
+                      d.listCount.global()[lb], L.acceptedByBlock + blockIdx.x, sSum);
+}
+
+// Pass 2 of a single system with several lists per workgroup: each group
+// of `group` threads (whole waves) walks one pass-1 list, DEPTH entries per
+// thread at a time.  Workgroup q runs on XCD q % 8 and takes lists of that
+// XCD's share (the lists its pass-1 workgroups wrote, whose lines that XCD's
+// L2 holds).  The list count and a thread's first entries do not depend on
+// the state, so they load with it: the walk then waits for two round trips,
+// the state with the list, then the proposal words of DEPTH moves at once.
+template <int LAT, int DEPTH>
+__global__ __launch_bounds__(256) void k_perform_lists(SysDesc d, uint32_t k, uint32_t listStride,
+                                                       uint32_t listsPerXcd, uint32_t group) {
+    __shared__ uint32_t sSum[4];
+    const uint32_t xcd = blockIdx.x & 7u, local = blockIdx.x >> 3;
+    const uint32_t gi = threadIdx.x / group, lane = threadIdx.x - gi * group;
+    const uint32_t li = local * (blockDim.x / group) + gi;
+    const bool haveList = li < listsPerXcd;
+    const uint32_t lst = xcd * listsPerXcd + (haveList ? li : 0u);
+    const gptr<const uint64_t> list = (gptr<const uint64_t>)d.list.global() + size_t(lst) * listStride;
+    uint32_t count = d.listCount.global()[lst];
+    uint64_t e[DEPTH];
+#pragma unroll
+    for (int r = 0; r < DEPTH; ++r) e[r] = list[min(lane + uint32_t(r) * group, listStride - 1u)];
+    Launch L;
+    if (!begin_substep(d, k, L)) return;
+    if (!haveList) count = 0;
+    const uint64_t t = L.t;
+    const uint32_t slotOffset =
+        ((gptr<const uint32_t>)(d.colors.global() + L.draw.species))[offsetof(ColorInfo, slotOffset) / 4];
+    clear_other_parity(d, t, blockIdx.x, gridDim.x);
+    const AxisMasks masks = axis_masks(d);
+    const gptr<const uint64_t> T64 =
+        (gptr<const uint64_t>)(d.proposal.global() + size_t(t & 1u) * d.latWords);
+    const gptr<uint32_t> pos = d.pos.global();
+    const gptr<uint8_t> lat = d.lattice.global();
+
+    uint32_t accepted = 0;
+    for (uint32_t base = 0; base < count; base += DEPTH * group) {  // count is the same for a whole wave
+        if (base != 0) {
+#pragma unroll
+            for (int r = 0; r < DEPTH; ++r) e[r] = list[min(base + lane + uint32_t(r) * group, listStride - 1u)];
+        }
+        // Entries past the count are stale: their lanes test the plane of
+        // code 0 (in bounds) and drop the result.
+        Move mv[DEPTH];
+        uint32_t hit[DEPTH];
+        bool valid[DEPTH];
+#pragma unroll
+        for (int r = 0; r < DEPTH; ++r) {
+            valid[r] = base + lane + uint32_t(r) * group < count;
+            const uint32_t m = valid[r] ? uint32_t(e[r] >> 32) : 0u;
+            const uint32_t dir = valid[r] ? uint32_t(e[r]) >> kDirShift : 0u;
+            mv[r] = make_move(m, dir, masks);
+            hit[r] = plane_any<6>(T64, mv[r]);
+        }
+#pragma unroll
+        for (int r = 0; r < DEPTH; ++r) {
+            if (!valid[r] || hit[r]) continue;
+            const uint32_t jj = uint32_t(e[r]) & ((1u << kDirShift) - 1u);
+            apply_move<LAT>(d, pos, lat, slotOffset + jj, uint32_t(e[r] >> 32), uint32_t(e[r]) >> kDirShift, mv[r]);
+            ++accepted;
+        }
+    }
+    const uint32_t total = block_sum(accepted, sSum);
+    if (threadIdx.x == 0 && total != 0u) {
+        L.acceptedByBlock[blockIdx.x] += total;
+        if (L.log != nullptr) {
+            const uint64_t idx = t - L.logFirst;
+            if (idx < L.logCapacity)
+                __hip_atomic_fetch_add(&L.log[idx], total, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+    }
 }
 
 // ---- batch of replicas ---------------------------------------------------------------
@@ -461,7 +541,7 @@ __global__ __launch_bounds__(256) void k_perform_batch(const SysDesc* __restrict
     if (!begin_substep(d, k, L)) return;
     perform_body<LAT>(d, L, tile.chunk, tile.chunks,
                       global_ptr(reinterpret_cast<const uint64_t*>(lists)) + size_t(blockIdx.x) * listStride,
-                      global_ptr(listCounts)[blockIdx.x], global_ptr(L.acceptedByBlock) + tile.chunk, sSum);
+                      global_ptr(listCounts)[blockIdx.x], L.acceptedByBlock + tile.chunk, sSum);
 }
 
 __global__ void k_advance(DevState* st, uint32_t count) {
@@ -473,7 +553,7 @@ __global__ void k_advance(DevState* st, uint32_t count) {
 __global__ void k_advance_batch(const SysDesc* descs, uint32_t nDescs, uint32_t count) {
     const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= nDescs) return;
-    DevState* st = descs[r].state;
+    const gptr<DevState> st = descs[r].state.global();
     if (st == nullptr) return;
     const uint64_t next = st->substep + count;
     st->substep = next < st->stopAt ? next : (st->substep > st->stopAt ? st->substep : st->stopAt);
@@ -481,8 +561,10 @@ __global__ void k_advance_batch(const SysDesc* descs, uint32_t nDescs, uint32_t 
 
 __global__ void k_set_stops(const SysDesc* descs, uint32_t nDescs, const uint64_t* stops) {
     const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= nDescs || descs[r].state == nullptr) return;
-    descs[r].state->stopAt = stops[r];
+    if (r >= nDescs) return;
+    const gptr<DevState> st = descs[r].state.global();
+    if (st == nullptr) return;
+    st->stopAt = stops[r];
 }
 
 // One workgroup per replica sums its per-workgroup accepted counters.
@@ -490,10 +572,10 @@ __global__ void k_gather_state(const SysDesc* descs, uint32_t nDescs, uint64_t* 
     __shared__ uint64_t part[256];
     const uint32_t r = blockIdx.x;
     if (r >= nDescs) return;
-    const DevState* st = descs[r].state;
+    const gptr<const DevState> st = descs[r].state.global();
     uint64_t sum = 0;
     if (st != nullptr)
-        for (uint32_t i = threadIdx.x; i < descs[r].accEntries; i += blockDim.x) sum += st->acceptedByBlock[i];
+        for (uint32_t i = threadIdx.x; i < descs[r].accEntries; i += blockDim.x) sum += global_ptr(st->acceptedByBlock)[i];
     part[threadIdx.x] = sum;
     __syncthreads();
     for (uint32_t w = blockDim.x / 2; w > 0; w >>= 1) {
@@ -504,6 +586,33 @@ __global__ void k_gather_state(const SysDesc* descs, uint32_t nDescs, uint64_t* 
         out[3 * r + 0] = st ? st->substep : 0;
         out[3 * r + 1] = st ? st->stopAt : 0;
         out[3 * r + 2] = part[0];
+    }
+}
+
+// The bits of m selected by `mask`, packed from bit 0 up (mask is the same
+// for every lane, so the loop is uniform).
+__device__ __forceinline__ uint32_t compact_bits(uint32_t m, uint32_t mask) {
+    uint32_t v = 0, k = 0;
+    while (mask != 0u) {
+        const uint32_t b = uint32_t(__builtin_ctz(mask));
+        v |= ((m >> b) & 1u) << k;
+        ++k;
+        mask &= mask - 1u;
+    }
+    return v;
+}
+
+__global__ void k_unfold(SysDesc d, const uint32_t* __restrict__ slotOf, uint32_t n, int32_t* __restrict__ xyz) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t s = slotOf[i];
+    const uint32_t m = d.pos.global()[s];
+    const gptr<const int32_t> img[3] = {d.imgX.global(), d.imgY.global(), d.imgZ.global()};
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const uint32_t mask = d.mask[a];
+        const int32_t edge = int32_t(1u << __builtin_popcount(mask));
+        xyz[3 * size_t(i) + a] = int32_t(compact_bits(m, mask)) + img[a][s] * edge;
     }
 }
 
@@ -557,8 +666,14 @@ void perform_launch(const SysDesc& d, const BatchLaunch* b, uint32_t k, const La
     if (b)
         hipLaunchKernelGGL((k_perform_batch<LAT>), dim3(b->tiles), blk, 0, st, b->descs, b->tileTable, b->lists,
                            b->listCounts, k, s.listStride());
-    else
+    else if (s.listGroup == 0)
         hipLaunchKernelGGL((k_perform<LAT>), dim3(s.grid), blk, 0, st, d, k, s.listStride());
+    else if (s.listDepth >= 4)
+        hipLaunchKernelGGL((k_perform_lists<LAT, 4>), dim3(s.passTwoGrid()), blk, 0, st, d, k, s.listStride(),
+                           s.listsPerXcd(), s.listGroup);
+    else
+        hipLaunchKernelGGL((k_perform_lists<LAT, 1>), dim3(s.passTwoGrid()), blk, 0, st, d, k, s.listStride(),
+                           s.listsPerXcd(), s.listGroup);
 }
 
 }  // namespace
@@ -617,6 +732,12 @@ hipError_t launch_set_stops(const SysDesc* descs, uint32_t nDescs, const uint64_
 hipError_t launch_gather_state(const SysDesc* descs, uint32_t nDescs, uint64_t* out, hipStream_t st) {
     if (nDescs == 0) return hipSuccess;
     hipLaunchKernelGGL(k_gather_state, dim3(nDescs), dim3(256), 0, st, descs, nDescs, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_unfold(const SysDesc& d, const uint32_t* slotOf, uint32_t n, int32_t* xyz, hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_unfold, dim3((n + 255) / 256), dim3(256), 0, st, d, slotOf, n, xyz);
     return hipGetLastError();
 }
 

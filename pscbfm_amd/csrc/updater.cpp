@@ -220,9 +220,30 @@ void Updater::setPerformBlock(uint32_t block) {
     if (initialized_) applyLaunchShape();
 }
 
+void Updater::setPerformWalk(uint32_t group, uint32_t depth) {
+    if (group != kLegacyWalk && group != 0 && (group % 64 != 0 || group > 256))
+        throw std::invalid_argument("[setPerformWalk] threads per list must be 64, 128, 192 or 256");
+    // Depth 2 is not offered: its uint8-lattice build gave wrong results on
+    // gfx950 at the 512^3 melt, while depths 1 and 4 are exact.
+    if (depth != 0 && depth != 1 && depth != 4)
+        throw std::invalid_argument("[setPerformWalk] entries per thread must be 1 or 4");
+    walkGroupOverride_ = group;
+    walkDepthOverride_ = depth;
+    if (initialized_) applyLaunchShape();
+}
+
 void Updater::applyLaunchShape() {
     shape_.block = blockOverride_ ? blockOverride_ : 256;
     shape_.slots = slotsOverride_ ? slotsOverride_ : 2;
+    // Pass 2 walks one list per workgroup unless a list group is set: on the
+    // 512^3 melt the multi-list walk was no faster (128 x 1: 89.5 against
+    // 89.3 G attempts/s; 64 x 4: 84.1), pass 2 being bound by its writes.
+    if (walkGroupOverride_ == 0 || walkGroupOverride_ == kLegacyWalk) {
+        shape_.listGroup = 0;
+    } else {
+        shape_.listGroup = walkGroupOverride_;
+        shape_.listDepth = walkDepthOverride_ ? walkDepthOverride_ : 4;
+    }
     // Pass 2 walks one compacted pass-1 list per workgroup and clears one
     // slice of the other-parity proposal lattice.  In a dense system (at
     // least a quarter of the cells occupied) the lists hold about a fifth of
@@ -234,8 +255,14 @@ void Updater::applyLaunchShape() {
     const double occupied = cells_ ? 8.0 * double(n_) / double(cells_) : 0.0;
     shape_.performBlock = performOverride_ ? performOverride_
                                            : (occupied >= 0.25 && shape_.block > 128u ? 128u : 0u);
+    if (shape_.listGroup) {
+        // several lists per workgroup: whole lists, at most 256 threads
+        if (!performOverride_) shape_.performBlock = 256;
+        if (shape_.listGroup > shape_.passTwoBlock()) shape_.listGroup = shape_.passTwoBlock();
+        shape_.performBlock = shape_.passTwoBlock() / shape_.listGroup * shape_.listGroup;
+    }
     shape_.grid = grid_for(layout_.maxColorCount, shape_.block * shape_.slots);
-    performGridMax_ = std::max(performGridMax_, shape_.grid);
+    performGridMax_ = std::max(performGridMax_, shape_.passTwoGrid());
     dropGraphs();
 }
 
@@ -508,6 +535,7 @@ void Updater::runChunk(uint64_t S, uint32_t* log, bool profile, bool useGraph) {
     // Species and kernel seed of every substep, as the kernels will draw them.
     std::vector<uint32_t> species(S);
     std::vector<uint32_t> draws;
+    uint32_t* drawsOnDevice = nullptr;
     if (r250Mode_) {
         // CRITICAL.cu:509-510: species first, then the kernel seed
         draws.resize(2 * S);
@@ -523,7 +551,9 @@ void Updater::runChunk(uint64_t S, uint32_t* log, bool profile, bool useGraph) {
             hip_check(hipMalloc(&drawTable_, S * 8), "hipMalloc (draw table)");
             drawCapacity_ = S;
         }
-        hip_check(hipMemcpyAsync(drawTable_, draws.data(), S * 8, hipMemcpyHostToDevice, stream_), "draw table H2D");
+        drawsOnDevice = drawTable_;
+        hip_check(hipMemcpyAsync(drawsOnDevice, draws.data(), S * 8, hipMemcpyHostToDevice, stream_),
+                  "draw table H2D");
     } else {
         for (uint64_t k = 0; k < S; ++k) species[k] = substep_draw(seed_, substep_ + k, nc).species;
     }
@@ -543,18 +573,18 @@ void Updater::runChunk(uint64_t S, uint32_t* log, bool profile, bool useGraph) {
         }
         if (r250Mode_) {
             // the table covers substeps [substep_, substep_ + S); none beyond runs
-            patch.drawTable = drawTable_;
+            patch.drawTable = drawsOnDevice;
             patch.drawBase = substep_;
             patch.stopAt = substep_ + S;
         }
         dev_.writeState(patch, stream_);
     }
 
-    hipEvent_t e0, e1;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
     hip_check(hipEventCreate(&e0), "hipEventCreate");
     hip_check(hipEventCreate(&e1), "hipEventCreate");
-    std::vector<hipEvent_t> kev;
     hip_check(hipEventRecord(e0, stream_), "hipEventRecord");
+    std::vector<hipEvent_t> kev;
     if (profile) {
         kev.resize(3 * S);
         for (auto& e : kev) hip_check(hipEventCreate(&e), "hipEventCreate");
@@ -604,8 +634,8 @@ void Updater::runChunk(uint64_t S, uint32_t* log, bool profile, bool useGraph) {
         lastRun_.performLaunches = S;
         for (auto& e : kev) (void)hipEventDestroy(e);
     }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
 
     // Attempts and bond terms follow from the host copy of the draws: every
     // member of the drawn species is attempted, with all its bonds.

@@ -95,6 +95,11 @@ public:
     // Pass-2 threads per workgroup (0 = automatic: 128 for systems at least a
     // quarter occupied, else the pass-1 block).
     void setPerformBlock(uint32_t block);
+    // Pass-2 walk: `group` threads per pass-1 list (64..256, multiple of 64)
+    // with `depth` entries per thread in flight (1 or 4; 0 = 4); group 0 or
+    // kLegacyWalk (the default) walks one list per workgroup.
+    static constexpr uint32_t kLegacyWalk = 0xFFFFFFFFu;
+    void setPerformWalk(uint32_t group, uint32_t depth);
     // R250 mode: from the next run on, every substep draws its species
     // (r250 % nColors) and kernel seed (r250) from this generator in the
     // reference's order, instead of the counter draw of (seed, substep).
@@ -190,6 +195,7 @@ private:
     bool checkOnInit_ = true;
     uint32_t graphChunk_ = 0;   // 0: choose automatically
     uint32_t blockOverride_ = 0, slotsOverride_ = 0, performOverride_ = 0;
+    uint32_t walkGroupOverride_ = 0, walkDepthOverride_ = 0;
 
     // device
     bool initialized_ = false;

@@ -194,3 +194,26 @@ def test_dense_melt_relaxed_on_both_lattices_bit_exact():
     _, olog = o.run(t0, 3 * (int(colors.max()) + 1))
     assert np.array_equal(glog, olog)
     assert np.array_equal(gx, o.xyz)
+
+
+@pytest.mark.parametrize("kind", KINDS)
+@pytest.mark.parametrize("group,depth,block,slots", [
+    (64, 1, 0, 0), (64, 4, 0, 0), (128, 1, 0, 0), (128, 4, 0, 0), (256, 1, 0, 0), (256, 4, 0, 0),
+    (P.LEGACY_WALK, 0, 0, 0),
+    (64, 4, 64, 1),     # pass-1 lists of 64 entries: the speculative reads are clamped to the list
+    (128, 4, 128, 4),
+])
+def test_perform_walk_does_not_change_results(kind, group, depth, block, slots):
+    # a dilute system (occupied fraction 0.04): lists fill to about half their
+    # capacity, so a list group walks several rounds of entries
+    s = chains((64, 64, 64), 40, 64, seed=11)
+    u = make_updater(s, 5, kind=kind, block=block, slots=slots)
+    u.setPerformWalk(group, depth)
+    _, log = u.run(15, log=True)
+    x = u.getPositions(s.n)
+    colors = u.colors(s.n)
+    u.cleanup()
+    from helpers import oracle_for
+    o = oracle_for(s, colors, 5)
+    _, olog = o.run(0, 15 * (int(colors.max()) + 1))
+    assert np.array_equal(log, olog) and np.array_equal(x, o.xyz)

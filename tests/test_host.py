@@ -503,3 +503,34 @@ def test_crosslinked_network_has_partner_escapes():
     partners, escapes = _staged_without_gpu(s).partnerEscapes()
     assert partners == 2 * s.bonds.shape[0]
     assert 0 < escapes < partners
+
+
+def test_perform_walk_is_validated():
+    u = P.UpdaterGPUScBFM_AB_Type()
+    u.setPerformWalk(0, 0)
+    u.setPerformWalk(128, 4)
+    u.setPerformWalk(P.LEGACY_WALK, 0)
+    for group, depth in ((32, 4), (320, 4), (64, 2), (64, 3), (64, 8)):
+        with pytest.raises(P.InvalidArgument):
+            u.setPerformWalk(group, depth)
+
+
+def test_device_code_has_no_flat_memory_instructions(tmp_path):
+    # Every device buffer is reached through a global-address-space pointer
+    # (descriptor pointers are DeviceBuffer, readable on the device only as
+    # global()); no kernel may fall back to generic FLAT loads or stores.
+    hipcc = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
+    if not Path(hipcc).exists():
+        pytest.skip("hipcc not available")
+    csrc = ROOT / "pscbfm_amd" / "csrc"
+    asm = tmp_path / "kernels.s"
+    for src in sorted(csrc.glob("*.hip")):
+        r = subprocess.run([hipcc, "-O3", "-std=c++17", "-x", "hip", "--offload-arch=gfx950", "-Xclang",
+                            "-target-feature", "-Xclang", "-packed-fp32-ops", "--cuda-device-only", "-S",
+                            "-I", str(csrc), "-I", str(ROOT / "include"), str(src), "-o", str(asm)],
+                           capture_output=True, text=True)
+        assert r.returncode == 0, r.stderr
+        text = asm.read_text()
+        assert re.search(r"^\s+global_load", text, re.M) or src.name != "kernels.hip"
+        flat = re.findall(r"^\s+(flat_(?:load|store|atomic)\w*)", text, re.M)
+        assert not flat, f"{src.name}: {sorted(set(flat))}"
