@@ -182,6 +182,17 @@ typedef struct pscbfm_run_stats {
 int pscbfm_run_ex(pscbfm_updater* h, int32_t n_mcs, uint32_t* log, int profile, int use_graph,
                   pscbfm_run_stats* stats);
 int pscbfm_last_run_stats(const pscbfm_updater* h, pscbfm_run_stats* stats);
+/* pscbfm_run_ex(h, n_mcs, log, 0, 0, stats) without any device allocation.
+ * No counterpart in the reference: it is for callers that cannot allocate
+ * device memory (code under graph capture, serving loops, harnesses that
+ * forbid hipMalloc).  The call's device log and, in R250 mode, its draw table
+ * live in a static scratch array of 3 MiB that each GPU the library runs on
+ * holds until the process ends, shared by every such call on that GPU: calls
+ * that overlap on one GPU race.  With a log or in R250 mode a call covers at
+ * most 262144 substeps (n_mcs * n_colors); beyond that it returns
+ * PSCBFM_INVALID_ARGUMENT.  Launches are eager; stats->gpu_ms is host time
+ * from the first launch to completion.  Results equal pscbfm_run_ex's. */
+int pscbfm_run_ex_noalloc(pscbfm_updater* h, int32_t n_mcs, uint32_t* log, pscbfm_run_stats* stats);
 
 /* Layout introspection: number of species, per-monomer species, species sizes. */
 int pscbfm_get_n_colors(const pscbfm_updater* h, uint32_t* n_colors);

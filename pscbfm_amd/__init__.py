@@ -138,6 +138,7 @@ _SIGNATURES = {
     "pscbfm_prepare_graphs": (C.c_int, [_H, C.c_int32]),
     "pscbfm_run_ex": (C.c_int, [_H, C.c_int32, _u32p, C.c_int, C.c_int, C.POINTER(RunStats)]),
     "pscbfm_last_run_stats": (C.c_int, [_H, C.POINTER(RunStats)]),
+    "pscbfm_run_ex_noalloc": (C.c_int, [_H, C.c_int32, _u32p, C.POINTER(RunStats)]),
     "pscbfm_get_n_colors": (C.c_int, [_H, _u32p]),
     "pscbfm_get_colors": (C.c_int, [_H, _u8p, C.c_uint32]),
     "pscbfm_get_color_sizes": (C.c_int, [_H, _u32p, C.c_uint32]),
@@ -399,6 +400,15 @@ class UpdaterGPUScBFM_AB_Type:
         buf = np.zeros(int(n_mcs) * n_colors, dtype=np.uint32) if log else None
         self._call("pscbfm_run_ex", int(n_mcs), _ptr(buf, C.c_uint32) if log else None,
                    int(bool(profile)), int(bool(use_graph)), C.byref(stats))
+        return stats, buf
+
+    def runNoAlloc(self, n_mcs: int, log: bool = False):
+        """run(n_mcs, log, use_graph=False) without any device allocation
+        (pscbfm_run_ex_noalloc); returns (RunStats, log or None)."""
+        stats = RunStats()
+        n_colors = self.nColors() if log else 0
+        buf = np.zeros(int(n_mcs) * n_colors, dtype=np.uint32) if log else None
+        self._call("pscbfm_run_ex_noalloc", int(n_mcs), _ptr(buf, C.c_uint32) if log else None, C.byref(stats))
         return stats, buf
 
     def autotune(self, mcs: int = 20):

@@ -286,6 +286,13 @@ int pscbfm_set_launch_shape(pscbfm_updater* h, uint32_t block, uint32_t slots) {
     return guarded(h, [&](pscbfm::Updater& u) { u.setLaunchShape(block, slots); });
 }
 
+int pscbfm_run_ex_noalloc(pscbfm_updater* h, int32_t n_mcs, uint32_t* log, pscbfm_run_stats* stats) {
+    return guarded(h, [&](pscbfm::Updater& u) {
+        u.runNoAlloc(n_mcs, log);
+        if (stats) fill_stats(u.lastRun(), stats);
+    });
+}
+
 int pscbfm_set_perform_walk(pscbfm_updater* h, uint32_t threads_per_list, uint32_t entries_per_thread) {
     return guarded(h, [&](pscbfm::Updater& u) { u.setPerformWalk(threads_per_list, entries_per_thread); });
 }

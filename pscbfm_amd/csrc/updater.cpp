@@ -2,11 +2,14 @@
 #include "updater.h"
 
 #include <algorithm>
+#include <chrono>
 #include <cstdlib>
 #include <cstring>
 #include <limits>
 #include <sstream>
 #include <stdexcept>
+
+#include "noalloc.h"
 
 namespace pscbfm {
 
@@ -483,6 +486,31 @@ void Updater::prepareGraphs(int32_t nMCS) {
 }
 
 void Updater::run(int32_t nMCS, uint32_t* log, bool profile, bool useGraph) {
+    runImpl(nMCS, log, profile, useGraph, nullptr);
+}
+
+// The allocation-free form: the device log and the R250 draw table of the
+// call live in the process's scratch array on this GPU (noalloc.hip) instead
+// of hipMalloc'd buffers, and every substep is launched eagerly (a graph's
+// instantiation allocates).
+void Updater::runNoAlloc(int32_t nMCS, uint32_t* log) {
+    if (!initialized_)
+        throw std::runtime_error("[runNoAlloc] initialize has not been called");
+    if (nMCS <= 0 || nColors() == 0 || n_ == 0) {
+        lastRun_ = RunStats{};
+        return;
+    }
+    const uint64_t S = uint64_t(nMCS) * nColors();
+    if ((log != nullptr || r250Mode_) && S > kNoAllocMaxSubsteps) {
+        std::ostringstream msg;
+        msg << "[runNoAlloc] " << S << " substeps exceed the scratch maximum of " << kNoAllocMaxSubsteps
+            << " substeps per call with a log or R250 draws";
+        throw std::invalid_argument(msg.str());
+    }
+    runImpl(nMCS, log, false, false, noalloc_scratch(gpu_));
+}
+
+void Updater::runImpl(int32_t nMCS, uint32_t* log, bool profile, bool useGraph, void* scratch) {
     if (!initialized_)
         throw std::runtime_error("[UpdaterGPUScBFM_AB_Type::runSimulationOnGPU] initialize has not been called");
     lastRun_ = RunStats{};
@@ -494,11 +522,11 @@ void Updater::run(int32_t nMCS, uint32_t* log, bool profile, bool useGraph) {
     // R250 mode keeps the draw table of one chunk of whole sweeps on the
     // device at a time (at most kMaxDrawSubsteps substeps, 8 bytes each).
     const uint64_t drawCap = drawChunk_ ? std::min(drawChunk_, kMaxDrawSubsteps) : kMaxDrawSubsteps;
-    const uint64_t chunk = r250Mode_ ? std::max<uint64_t>(1u, drawCap / nc) * nc : S;
+    const uint64_t chunk = r250Mode_ && !scratch ? std::max<uint64_t>(1u, drawCap / nc) * nc : S;
     RunStats total{};
     for (uint64_t done = 0; done < S; done += chunk) {
         const uint64_t part = std::min(chunk, S - done);
-        runChunk(part, log ? log + done : nullptr, profile, useGraph);
+        runChunk(part, log ? log + done : nullptr, profile, useGraph, scratch);
         total.gpuMilliseconds += lastRun_.gpuMilliseconds;
         total.substeps += lastRun_.substeps;
         total.attempted += lastRun_.attempted;
@@ -521,8 +549,11 @@ void Updater::run(int32_t nMCS, uint32_t* log, bool profile, bool useGraph) {
     if (st.substep != substep_) throw std::runtime_error("device substep counter out of step with the host");
 }
 
-// S substeps from substep_ on; fills lastRun_ except `accepted`.
-void Updater::runChunk(uint64_t S, uint32_t* log, bool profile, bool useGraph) {
+// S substeps from substep_ on; fills lastRun_ except `accepted`.  With
+// `scratch` (the allocation-free form) the draw table sits at its start and
+// the device log kNoAllocDrawBytes in, nothing is allocated, and the launches
+// are eager and timed on the host.
+void Updater::runChunk(uint64_t S, uint32_t* log, bool profile, bool useGraph, void* scratch) {
     lastRun_ = RunStats{};
     const uint32_t nc = nColors();
 
@@ -544,14 +575,14 @@ void Updater::runChunk(uint64_t S, uint32_t* log, bool profile, bool useGraph) {
             draws[2 * k + 1] = r250_.next();
             species[k] = draws[2 * k];
         }
-        if (drawCapacity_ < S) {
+        if (!scratch && drawCapacity_ < S) {
             if (drawTable_) (void)hipFree(drawTable_);
             drawTable_ = nullptr;
             drawCapacity_ = 0;
             hip_check(hipMalloc(&drawTable_, S * 8), "hipMalloc (draw table)");
             drawCapacity_ = S;
         }
-        drawsOnDevice = drawTable_;
+        drawsOnDevice = scratch ? static_cast<uint32_t*>(scratch) : drawTable_;
         hip_check(hipMemcpyAsync(drawsOnDevice, draws.data(), S * 8, hipMemcpyHostToDevice, stream_),
                   "draw table H2D");
     } else {
@@ -560,7 +591,10 @@ void Updater::runChunk(uint64_t S, uint32_t* log, bool profile, bool useGraph) {
 
     uint32_t* dLog = nullptr;
     if (log) {
-        hip_check(hipMalloc(&dLog, S * 4), "hipMalloc (log)");
+        if (scratch)  // the last call's counts are still there: cleared by a kernel
+            dLog = reinterpret_cast<uint32_t*>(static_cast<char*>(scratch) + kNoAllocDrawBytes);
+        else
+            hip_check(hipMalloc(&dLog, S * 4), "hipMalloc (log)");
         hip_check(launch_fill_u32(dLog, S, 0u, stream_), "fill log");
     }
     const bool patched = log != nullptr || r250Mode_;
@@ -580,10 +614,15 @@ void Updater::runChunk(uint64_t S, uint32_t* log, bool profile, bool useGraph) {
         dev_.writeState(patch, stream_);
     }
 
+    // The allocation-free form is timed on the host (launch to completion)
+    // and creates no events.
     hipEvent_t e0 = nullptr, e1 = nullptr;
-    hip_check(hipEventCreate(&e0), "hipEventCreate");
-    hip_check(hipEventCreate(&e1), "hipEventCreate");
-    hip_check(hipEventRecord(e0, stream_), "hipEventRecord");
+    const auto h0 = std::chrono::steady_clock::now();
+    if (!scratch) {
+        hip_check(hipEventCreate(&e0), "hipEventCreate");
+        hip_check(hipEventCreate(&e1), "hipEventCreate");
+        hip_check(hipEventRecord(e0, stream_), "hipEventRecord");
+    }
     std::vector<hipEvent_t> kev;
     if (profile) {
         kev.resize(3 * S);
@@ -617,11 +656,17 @@ void Updater::runChunk(uint64_t S, uint32_t* log, bool profile, bool useGraph) {
             left -= blk;
         }
     }
-    hip_check(hipEventRecord(e1, stream_), "hipEventRecord");
-    hip_check(hipEventSynchronize(e1), "hipEventSynchronize");
-    float ms = 0;
-    hip_check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
-    lastRun_.gpuMilliseconds = ms;
+    if (scratch) {
+        hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+        lastRun_.gpuMilliseconds =
+            std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - h0).count();
+    } else {
+        hip_check(hipEventRecord(e1, stream_), "hipEventRecord");
+        hip_check(hipEventSynchronize(e1), "hipEventSynchronize");
+        float ms = 0;
+        hip_check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+        lastRun_.gpuMilliseconds = ms;
+    }
     if (profile) {
         for (uint64_t k = 0; k < S; ++k) {
             float a = 0, b = 0;
@@ -658,7 +703,7 @@ void Updater::runChunk(uint64_t S, uint32_t* log, bool profile, bool useGraph) {
         patch.stopAt = std::numeric_limits<uint64_t>::max();
         dev_.writeState(patch, stream_);
     }
-    if (dLog) (void)hipFree(dLog);
+    if (dLog && !scratch) (void)hipFree(dLog);
 }
 
 void Updater::setR250State(const uint32_t* words250) {

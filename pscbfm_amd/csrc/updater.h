@@ -118,6 +118,13 @@ public:
     // counts, one per substep.  profile=true launches eagerly with an event
     // pair around every kernel to time pass 1 and pass 2 separately.
     void run(int32_t nMCS, uint32_t* log, bool profile, bool useGraph);
+    // run(nMCS, log, false, false) without any device allocation: the device
+    // log and the R250 draw table of the call live in this process's scratch
+    // array on the updater's GPU (noalloc.h), shared by every such call on
+    // that GPU, so calls that overlap on one GPU race.  With a log or in R250
+    // mode a call may cover at most kNoAllocMaxSubsteps substeps
+    // (invalid_argument beyond).  Eager launches; gpuMilliseconds is host time.
+    void runNoAlloc(int32_t nMCS, uint32_t* log);
 
     uint32_t nMonomers() const { return n_; }
     uint32_t nColors() const { return uint32_t(layout_.colors.size()); }
@@ -156,7 +163,8 @@ private:
     void requireMonomers(const char* who) const;
     void requireIndex(uint32_t i, const char* who) const;
     void freeDevice();
-    void runChunk(uint64_t S, uint32_t* log, bool profile, bool useGraph);
+    void runImpl(int32_t nMCS, uint32_t* log, bool profile, bool useGraph, void* scratch);
+    void runChunk(uint64_t S, uint32_t* log, bool profile, bool useGraph, void* scratch);
     // Captured graph of `substeps` substeps plus their counter advance,
     // instantiated and uploaded on first use.
     hipGraphExec_t graphFor(uint32_t substeps);
