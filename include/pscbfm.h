@@ -122,6 +122,13 @@ int pscbfm_r250_get_state(const pscbfm_updater* h, uint32_t* words);
  * also the maximum).  A longer run goes through in chunks of whole sweeps,
  * with the same result. */
 int pscbfm_set_draw_chunk(pscbfm_updater* h, uint64_t substeps);
+/* Whether pscbfm_run / pscbfm_run_ex copy the positions back before they
+ * return (on = 1, the default, as runSimulationOnGPU does at
+ * CRITICAL.cu:673-695).  Off: the getters return the coordinates of the last
+ * read-back until pscbfm_read_positions_back, so that repeated runs whose
+ * coordinates are not needed in between skip the copy. */
+int pscbfm_set_read_back(pscbfm_updater* h, int on);
+int pscbfm_read_positions_back(pscbfm_updater* h);
 /* Occupancy lattice storage (pscbfm_lattice_kind).  Before initialize.
  * Replaces the USE_BIT_PACKING_LATTICE switch (UpdaterGPUScBFM_AB_Type.cu:13). */
 int pscbfm_set_lattice_kind(pscbfm_updater* h, int kind);

@@ -134,6 +134,8 @@ _SIGNATURES = {
     "pscbfm_set_launch_shape": (C.c_int, [_H, C.c_uint32, C.c_uint32]),
     "pscbfm_set_perform_block": (C.c_int, [_H, C.c_uint32]),
     "pscbfm_set_graph_chunk": (C.c_int, [_H, C.c_uint32]),
+    "pscbfm_set_read_back": (C.c_int, [_H, C.c_int]),
+    "pscbfm_read_positions_back": (C.c_int, [_H]),
     "pscbfm_set_perform_walk": (C.c_int, [_H, C.c_uint32, C.c_uint32]),
     "pscbfm_prepare_graphs": (C.c_int, [_H, C.c_int32]),
     "pscbfm_run_ex": (C.c_int, [_H, C.c_int32, _u32p, C.c_int, C.c_int, C.POINTER(RunStats)]),
@@ -372,6 +374,13 @@ class UpdaterGPUScBFM_AB_Type:
         """Pass-2 walk: threads per pass-1 list (0 or LEGACY_WALK, the default:
         one list per workgroup) and entries in flight per thread (1 or 4; 0: 4)."""
         self._call("pscbfm_set_perform_walk", int(threads_per_list), int(entries_per_thread))
+
+    def setReadBack(self, on: bool) -> None:
+        """Whether runs copy the positions back (default on, as runSimulationOnGPU)."""
+        self._call("pscbfm_set_read_back", int(bool(on)))
+
+    def readPositionsBack(self) -> None:
+        self._call("pscbfm_read_positions_back")
 
     def setGraphChunk(self, substeps: int) -> None:
         self._call("pscbfm_set_graph_chunk", int(substeps))

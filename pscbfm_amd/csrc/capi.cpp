@@ -297,6 +297,14 @@ int pscbfm_set_perform_walk(pscbfm_updater* h, uint32_t threads_per_list, uint32
     return guarded(h, [&](pscbfm::Updater& u) { u.setPerformWalk(threads_per_list, entries_per_thread); });
 }
 
+int pscbfm_set_read_back(pscbfm_updater* h, int on) {
+    return guarded(h, [&](pscbfm::Updater& u) { u.setReadBack(on != 0); });
+}
+
+int pscbfm_read_positions_back(pscbfm_updater* h) {
+    return guarded(h, [&](pscbfm::Updater& u) { u.readPositionsBack(); });
+}
+
 int pscbfm_set_graph_chunk(pscbfm_updater* h, uint32_t substeps) {
     return guarded(h, [&](pscbfm::Updater& u) { u.setGraphChunk(substeps); });
 }

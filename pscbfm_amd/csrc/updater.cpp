@@ -537,7 +537,7 @@ void Updater::runImpl(int32_t nMCS, uint32_t* log, bool profile, bool useGraph, 
         total.performLaunches += lastRun_.performLaunches;
     }
     lastRun_ = total;
-    syncFromDevice();
+    if (readBack_) syncFromDevice();
     const DevState st = dev_.readState(stream_);
     std::vector<uint64_t> perBlock(performGridMax_);
     hip_check(hipMemcpy(perBlock.data(), st.acceptedByBlock, perBlock.size() * sizeof(uint64_t), hipMemcpyDeviceToHost),
@@ -735,6 +735,12 @@ void Updater::r250State(uint32_t* words250) const {
 void Updater::runSimulationOnGPU(int32_t nMCS) { run(nMCS, nullptr, false, true); }
 
 void Updater::syncFromDevice() { dev_.positions(layout_, xyz_.data(), stream_); }
+
+void Updater::readPositionsBack() {
+    if (!initialized_) throw std::runtime_error("[readPositionsBack] initialize has not been called");
+    hip_check(hipSetDevice(gpu_), "hipSetDevice");
+    syncFromDevice();
+}
 
 void Updater::downloadLattice(std::vector<uint8_t>& out) const {
     if (!initialized_) throw std::runtime_error("[downloadLattice] not initialized");

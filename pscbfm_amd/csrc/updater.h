@@ -111,6 +111,11 @@ public:
     // Substeps of explicit draws kept on the device at once (0 = 2^22, 32 MiB);
     // a longer run goes through in chunks of whole sweeps.
     void setDrawChunk(uint64_t substeps) { drawChunk_ = substeps; }
+    // Whether a run copies the positions back before it returns (default on,
+    // as runSimulationOnGPU does).  Off: the getters keep the coordinates of
+    // the last read-back until readPositionsBack().
+    void setReadBack(bool on) { readBack_ = on; }
+    void readPositionsBack();
     void setMonomerCoordinatesBulk(const int32_t* xyz, uint32_t n);
     void setConnectivityBulk(const uint32_t* pairs, uint64_t nPairs);
     void getPositionsBulk(int32_t* xyz, uint32_t n) const;
@@ -197,6 +202,7 @@ private:
     uint64_t seed_ = 0;
     uint64_t startMcs_ = 0;
     bool r250Mode_ = false;
+    bool readBack_ = true;
     R250 r250_;
     uint32_t latticeKind_ = kLatticeBit;
     bool uniformColors_ = true;

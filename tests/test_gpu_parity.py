@@ -217,3 +217,21 @@ def test_perform_walk_does_not_change_results(kind, group, depth, block, slots):
     o = oracle_for(s, colors, 5)
     _, olog = o.run(0, 15 * (int(colors.max()) + 1))
     assert np.array_equal(log, olog) and np.array_equal(x, o.xyz)
+
+
+def test_runs_without_read_back_keep_the_last_coordinates():
+    # with the read-back off the getters keep the coordinates of the last
+    # read-back; readPositionsBack() then returns those of the run alone
+    s = chains((64, 64, 64), 40, 64, seed=11)
+    u = make_updater(s, 5)
+    before = u.getPositions(s.n)
+    u.setReadBack(False)
+    _, log1 = u.run(7, log=True)
+    assert np.array_equal(u.getPositions(s.n), before)
+    _, log2 = u.run(8, log=True)
+    u.readPositionsBack()
+    x = u.getPositions(s.n)
+    u.cleanup()
+    gx, glog, ox, olog = gpu_vs_oracle(s, seed=5, n_mcs=15)
+    assert np.array_equal(np.concatenate([log1, log2]), olog)
+    assert np.array_equal(x, ox)

@@ -534,3 +534,11 @@ def test_device_code_has_no_flat_memory_instructions(tmp_path):
         assert re.search(r"^\s+global_load", text, re.M) or src.name != "kernels.hip"
         flat = re.findall(r"^\s+(flat_(?:load|store|atomic)\w*)", text, re.M)
         assert not flat, f"{src.name}: {sorted(set(flat))}"
+
+
+def test_read_back_switch_before_initialize():
+    u = _filled(1)
+    u.setReadBack(False)
+    u.setReadBack(True)
+    with pytest.raises(RuntimeError, match="initialize"):
+        u.readPositionsBack()
