@@ -372,8 +372,14 @@ def main() -> int:
                     help="target seconds of the oracle sample that checks the other lattice kind")
     ap.add_argument("--eager-sample", type=int, default=64,
                     help="replicas of the plain loop re-run with eager launches")
-    ap.add_argument("--serve-rates", default="20,80", help="Poisson arrival rates (jobs/s) of the served stream")
+    ap.add_argument("--serve-rates", default="20,80,160,320",
+                    help="Poisson arrival rates (jobs/s) of the served stream")
     ap.add_argument("--serve-seconds", type=float, default=8.0, help="arrival window per rate")
+    ap.add_argument("--pressure-budget-gb", type=float, default=0.6,
+                    help="HBM budget of the stream where admission by footprint binds (0: skip it)")
+    ap.add_argument("--pressure-melts", type=int, default=6, help="512^3 melts in the budget-bound stream")
+    ap.add_argument("--pressure-melt-mcs", type=int, default=20000, help="sweeps each of those melts asks for")
+    ap.add_argument("--pressure-rate", type=float, default=20.0, help="arrival rate of the budget-bound stream")
     ap.add_argument("--no-serve", action="store_true")
     args = ap.parse_args()
 
@@ -596,7 +602,9 @@ def main() -> int:
                 f"({sum(j.kind == 'melt' for j in jobs)} dense 512^3 melts)")
             res = serving.run_stream(jobs, log=log, seed=args.seed)
             log(f"  {res['jobs_per_s']:.1f} jobs/s, latency p50 {res['latency_p50_s']:.2f} s "
-                f"p99 {res['latency_p99_s']:.2f} s, GPU busy {res['gpu_busy_fraction']:.2f}; {res['parity']}")
+                f"p99 {res['latency_p99_s']:.2f} s, queue wait p99 {res['queue_wait_p99_s']:.2f} s, GPU busy "
+                f"{res['gpu_busy_fraction']:.2f}, peak resident {res['peak_resident_GB']:.2f} GB; {res['parity']} "
+                f"(sweeps {res['verified_mcs']})")
             if res["melts"]:
                 log(f"  melt jobs ({res['melt_mcs']} MCS) done after {[round(x, 2) for x in res['melt_latency_s']]} s; "
                     f"{res['melt_parity']}")
@@ -604,6 +612,22 @@ def main() -> int:
             if plain:
                 res["speedup_vs_plain_loop"] = res["monomer_sweeps_per_s"] / plain
             out["serving"][f"{rate:g}/s"] = res
+        if args.pressure_budget_gb > 0:
+            # a stated budget small enough that admission by device footprint
+            # binds: several long 512^3 melts (0.25 GB each) among the jobs, so
+            # that a melt arriving while another is resident has to wait
+            rate = args.pressure_rate
+            jobs = serving.draw_jobs(rate, args.serve_seconds, args.seed + 99, melts=args.pressure_melts,
+                                     melt_mcs=args.pressure_melt_mcs)
+            budget = int(args.pressure_budget_gb * 1e9)
+            log(f"served stream at {rate:g} jobs/s under a {args.pressure_budget_gb:g} GB budget: {len(jobs)} jobs "
+                f"({sum(j.kind == 'melt' for j in jobs)} dense 512^3 melts)")
+            res = serving.run_stream(jobs, hbm_budget=budget, log=log, seed=args.seed)
+            log(f"  {res['jobs_per_s']:.1f} jobs/s, latency p50 {res['latency_p50_s']:.2f} s "
+                f"p99 {res['latency_p99_s']:.2f} s, queue wait p99 {res['queue_wait_p99_s']:.2f} s, "
+                f"peak resident {res['peak_resident_GB']:.3f} of {args.pressure_budget_gb:g} GB; {res['parity']}; "
+                f"{res['melt_parity']}")
+            out["serving"][f"{rate:g}/s, {args.pressure_budget_gb:g} GB budget"] = res
     print(json.dumps(out), flush=True)
     return 0
 

@@ -173,6 +173,8 @@ uint32_t Batch::add(const SystemLayout& L, uint64_t seed, uint64_t startMcs) {
     const uint32_t bucket = degree_bucket(r.dev->desc.maxDegree);
     if (bucket > graphDegree_) {
         graphDegree_ = bucket;
+        // an earlier launch of a cached exec may still be running
+        hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
         dropGraphs();
     }
     tablesDirty_ = true;
@@ -237,6 +239,9 @@ void Batch::uploadTables() {
     }
 }
 
+// Callers make sure no launch of a cached exec is still running: every path
+// that drops graphs (growTiles, setPerformBlock, add raising the degree
+// bucket, the destructor) synchronizes the stream first.
 void Batch::dropGraphs() {
     for (auto& kv : graphs_) (void)hipGraphExecDestroy(kv.second);
     graphs_.clear();

@@ -8,9 +8,10 @@ log-uniformly from [mcs_lo, mcs_hi].  Arrivals are a Poisson process of the
 given rate; a submitter thread hands each job to the server at its arrival
 time (the staging of the job's updater is done beforehand, so the arrival
 process is not delayed by host work).  Reported per rate: completed jobs/s,
-p50/p99 completion latency (submit -> result ready), p50 queueing delay,
-monomer-sweeps/s and the GPU busy fraction; a sample of the jobs is re-run
-alone in an updater and compared bit for bit.
+p50/p99 completion latency (submit -> result ready), p50/p99 queueing delay,
+monomer-sweeps/s, the GPU busy fraction and the peak resident device bytes
+against the budget; a random sample of the jobs, the longest jobs and every
+melt are re-run alone in an updater and compared bit for bit.
 """
 from __future__ import annotations
 
@@ -67,7 +68,7 @@ def draw_jobs(rate: float, duration: float, seed: int, mcs_lo: int = 1000, mcs_h
 
 def run_stream(jobs: List[Job], hbm_budget: int = 288 * 10**9, tile_capacity: int = 32768,
                replica_capacity: int = 8192, block: int = 256, slots: int = 2, quantum: int = 64,
-               lattice_kind: int = LATTICE_BIT, verify: int = 8, seed: int = 0,
+               lattice_kind: int = LATTICE_BIT, verify: int = 8, verify_longest: int = 2, seed: int = 0,
                log=None) -> dict:
     staged = [W.stage(j.system, lattice_kind) for j in jobs]
     srv = Server(hbm_budget, tile_capacity, replica_capacity, lattice_kind, block, slots, quantum)
@@ -95,10 +96,15 @@ def run_stream(jobs: List[Job], hbm_budget: int = 288 * 10**9, tile_capacity: in
     last = max(inf.t_done for inf in infos)
     span = last - first
     sweeps = sum(j.system.n * j.mcs for j in jobs)
-    # bit-exact check of a sample against the same system run alone
+    # bit-exact check against the same system run alone: a random sample of
+    # the replica jobs plus the longest ones (up to the 10^5-sweep end of the
+    # range), and every melt below
     rng = np.random.default_rng(seed)
-    small = [k for k, j in enumerate(jobs) if j.kind == "replica" and j.mcs <= 5000]
-    pick = sorted(rng.choice(small, size=min(verify, len(small)), replace=False).tolist()) if small else []
+    reps = [k for k, j in enumerate(jobs) if j.kind == "replica"]
+    longest = sorted(reps, key=lambda k: -jobs[k].mcs)[:verify_longest]
+    rest = [k for k in reps if k not in longest]
+    pick = sorted(rng.choice(rest, size=min(verify, len(rest)), replace=False).tolist()) if rest else []
+    pick = sorted(set(pick) | set(longest))
     def alone_equals_served(k: int) -> bool:
         u = W.stage(jobs[k].system, lattice_kind, initialize=True)
         u.runSimulationOnGPU(jobs[k].mcs)
@@ -117,12 +123,15 @@ def run_stream(jobs: List[Job], hbm_budget: int = 288 * 10**9, tile_capacity: in
         "latency_p50_s": float(np.percentile(lat, 50)),
         "latency_p99_s": float(np.percentile(lat, 99)),
         "queue_wait_p50_s": float(np.percentile(wait, 50)),
+        "queue_wait_p99_s": float(np.percentile(wait, 99)),
         "monomer_sweeps_per_s": sweeps / span,
         "gpu_busy_fraction": st.gpu_busy_seconds / span,
         "peak_resident_GB": st.peak_resident_bytes / 1e9,
+        "hbm_budget_GB": hbm_budget / 1e9,
         "launches": st.launches,
         "substeps": st.substeps,
         "parity": f"{same}/{len(pick)} sampled jobs bit-identical to their runs alone",
+        "verified_mcs": sorted(jobs[k].mcs for k in pick),
         "melt_latency_s": [float(lat[k]) for k in melt_ids],
         "melt_mcs": [jobs[k].mcs for k in melt_ids],
         "melt_parity": f"{melt_same}/{len(melt_ids)} melts bit-identical to their runs alone",
