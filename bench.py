@@ -159,7 +159,7 @@ def cpu_single_core(u, box, bonds, colors, seed, target_seconds: float) -> dict:
     return out
 
 
-def network_section(seed: int, mcs: int, kind: int, c_l: float) -> dict:
+def network_section(seed: int, mcs: int, kind: int, c_l: float, shape=None) -> dict:
     """Star network: 12800 4-arm stars of arm length 29 (1,497,600 monomers)
     with arm ends crosslinked to other stars, on a 512x256x256 periodic box
     (occupied fraction 0.357; the same monomer count on 256^3 would be 0.72,
@@ -175,9 +175,15 @@ def network_section(seed: int, mcs: int, kind: int, c_l: float) -> dict:
     n = xyz.shape[0]
     log(f"network: {n} monomers, {u.nColors()} species {u.colorSizes().tolist()}, "
         f"built in {time.perf_counter() - tic:.1f} s")
-    block, slots, trials = u.autotune(3)
-    log(f"network autotune: block {block} slots {slots}; "
-        + ", ".join(f"{b}x{s_}:{ms:.3f}" for (b, s_), ms in sorted(trials.items())))
+    if shape:
+        block, slots = shape
+        trials = {}
+        u.setLaunchShape(block, slots)
+        log(f"network shape fixed: block {block} slots {slots}")
+    else:
+        block, slots, trials = u.autotune(3)
+        log(f"network autotune: block {block} slots {slots}; "
+            + ", ".join(f"{b}x{s_}:{ms:.3f}" for (b, s_), ms in sorted(trials.items())))
     u.run(20)
     st, _ = u.run(mcs)
     p1, p2 = algorithmic_bytes(st, c_l)
@@ -216,7 +222,7 @@ def network_section(seed: int, mcs: int, kind: int, c_l: float) -> dict:
 
 
 def batched_vs_plain(n_rep: int, mcs: int, seed: int, kind: int, tune_sample: int, tune_mcs: int,
-                     plain: bool = True, shape=(0, 0), eager_sample: int = 64) -> dict:
+                     plain: bool = True, shape=(0, 0), eager_sample: int = 64, after_tune=None) -> dict:
     """Replica mix: one batch (autotuned shape, graph-replayed launches over
     every replica) against the plain loop of one updater per system, each
     called with runSimulationOnGPU(mcs) in turn.  Every replica's final
@@ -243,6 +249,11 @@ def batched_vs_plain(n_rep: int, mcs: int, seed: int, kind: int, tune_sample: in
         block, slots, trials = P.autotune_batch([staged[i] for i in pick], tune_mcs, kind)
         log(f"autotune on {len(pick)} replicas: block {block} slots {slots} ({time.perf_counter() - tic:.1f} s); "
             + ", ".join(f"{b}x{s_}:{ms:.3f}" for (b, s_), ms in sorted(trials.items())))
+        if after_tune:
+            # the tuner ran as usual; the batch takes the stated shape (profiled
+            # runs: the tuner's choice drifts under a profiler)
+            block, slots = after_tune
+            log(f"batch shape after tuning fixed at block {block} slots {slots}")
 
     batch = P.Batch(0, n_rep, kind, block, slots)
     tic = time.perf_counter()
@@ -366,6 +377,9 @@ def main() -> int:
     ap.add_argument("--no-plain", action="store_true", help="batched section without the plain per-system loop")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--network-mcs", type=int, default=2000)
+    ap.add_argument("--network-shape", default="", help="BLOCKxSLOTS of the network (default: autotuned)")
+    ap.add_argument("--batch-shape-after-tune", default="",
+                    help="BLOCKxSLOTS the batch uses after the autotuner has run (default: the tuner's pick)")
     ap.add_argument("--no-network", action="store_true")
     ap.add_argument("--compare-mcs", type=int, default=300, help="MCS of the other lattice kind on the same melt")
     ap.add_argument("--compare-cpu-seconds", type=float, default=3.0,
@@ -588,11 +602,15 @@ def main() -> int:
                 f"(acceptance {one['sequential_1_core']['acceptance']:.3f})")
     u.cleanup()
     if not args.no_network and args.network_mcs > 0:
-        out["network"] = network_section(args.seed, args.network_mcs, kind, c_l)
+        nshape = tuple(int(v) for v in args.network_shape.split("x")) if args.network_shape else None
+        out["network"] = network_section(args.seed, args.network_mcs, kind, c_l, nshape)
     if not args.no_batch and args.replicas > 0:
         out["batched"] = batched_vs_plain(args.replicas, args.replica_mcs, args.seed, kind, args.tune_sample,
                                           args.tune_mcs, plain=not args.no_plain,
-                                          shape=(args.batch_block, args.batch_slots))
+                                          shape=(args.batch_block, args.batch_slots),
+                                          eager_sample=args.eager_sample,
+                                          after_tune=tuple(int(v) for v in args.batch_shape_after_tune.split("x"))
+                                          if args.batch_shape_after_tune else None)
     if not args.no_serve and args.serve_rates:
         from pscbfm_amd import serving
         out["serving"] = {}
