@@ -235,3 +235,18 @@ def test_runs_without_read_back_keep_the_last_coordinates():
     gx, glog, ox, olog = gpu_vs_oracle(s, seed=5, n_mcs=15)
     assert np.array_equal(np.concatenate([log1, log2]), olog)
     assert np.array_equal(x, ox)
+
+
+@pytest.mark.parametrize("kind", KINDS)
+def test_network_workload_full_size_bit_exact(kind):
+    # The star-network workload of the benchmark at full size: 12800 4-arm
+    # stars of arm 29 (1,497,600 monomers) with arm ends crosslinked, in a
+    # 512x256x256 box; 2 MCS against the oracle.
+    s = stars((512, 256, 256), 12800, 4, 29, seed=42)
+    print("network full size", s.n, "kind", kind, flush=True)
+    from helpers import stage_updater
+    partners, escapes = stage_updater(s, 42, kind).partnerEscapes()
+    print("partners", partners, "escapes", escapes, flush=True)
+    gx, glog, ox, olog = gpu_vs_oracle(s, seed=42, n_mcs=2, kind=kind)
+    assert np.array_equal(glog, olog)
+    assert np.array_equal(gx, ox)

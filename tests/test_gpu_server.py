@@ -94,3 +94,25 @@ def test_job_larger_than_the_budget_fails_alone():
     with pytest.raises(RuntimeError):
         srv.job(job)
     srv.close()
+
+
+def test_long_job_beside_short_ones_equals_its_run_alone_and_the_oracle():
+    # a job of 20,000 sweeps shares the batch with short jobs that arrive,
+    # run and retire while it runs; its result is the one it has alone and
+    # on the CPU oracle
+    long_job = (chains((32, 32, 64), 16, 24, seed=13), 2024, 20000)
+    short = [(chains((32, 32, 32), 10, 16, seed=k), 100 + k, 50 + 10 * k) for k in range(6)]
+    srv = P.Server(hbm_budget=1 << 34, tile_capacity=256, replica_capacity=16, quantum=64)
+    s, seed, mcs = long_job
+    lid = srv.submit(stage_updater(s, seed), mcs)
+    sids = [(srv.submit(stage_updater(x, sd), m), x, sd, m) for x, sd, m in short]
+    srv.drain()
+    for jid, x, sd, m in sids:
+        assert np.array_equal(srv.result(jid, x.n), _alone(x, sd, m)[0])
+    got = srv.result(lid, s.n)
+    x, acc, colors = _alone(s, seed, mcs)
+    assert np.array_equal(got, x)
+    assert srv.job(lid).accepted == acc
+    ox, oacc = _cpu(s, seed, mcs, colors)
+    assert np.array_equal(got, ox) and acc == oacc
+    srv.close()
