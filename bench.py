@@ -483,6 +483,7 @@ def main() -> int:
              "k_perform": "k_perform" if args.lattice == "bit" else "k_perform.u8"}
     pdur = prof.get("durations", {})
     ptraf = (prof.get("traffic") or {}).get("kernels", {})
+    pcount = prof.get("counters", {})
 
     def kernel_entry(key, bytes_per_launch, mean_ms):
         e = {"algorithmic_bytes_per_launch": bytes_per_launch, "mean_launch_us_events": mean_ms * 1e3,
@@ -495,6 +496,12 @@ def main() -> int:
             e["frac_rocprof"] = e["achieved_GBps_rocprof"] / HBM_PEAK_GBS
         t = ptraf.get(names[key])
         e["traffic"] = t.get("traffic_bytes") if t else None
+        if t and d:
+            # measured HBM rate of the kernel: rocprof bytes over rocprof time
+            e["hbm_GBps_rocprof"] = t["traffic_bytes"] / (d["mean_us"] * 1e-6) / 1e9
+        c = pcount.get(names[key], {})
+        if c.get("TCC_HIT_sum", 0) + c.get("TCC_MISS_sum", 0) > 0:
+            e["l2_hit_rate_rocprof"] = c["TCC_HIT_sum"] / (c["TCC_HIT_sum"] + c["TCC_MISS_sum"])
         return e
 
     kernels = {"k_check (pass 1)": kernel_entry("k_check", k1_bytes, k1_mean_ms),
