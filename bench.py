@@ -426,12 +426,15 @@ def main() -> int:
     log(f"initialize done in {time.perf_counter() - tic:.1f} s; species sizes {u.colorSizes().tolist()}; "
         f"device bytes {u.deviceBytes() / 2**20:.0f} MiB")
 
-    # The timed run's graphs first, then the pre-relaxation (part of building
-    # the workload, whatever --warmup says) and the warm-up back to back with
-    # the timed run: those two skip the coordinate read-back, so the GPU runs
-    # without a gap up to the timed window (after an idle gap of a few ms its
-    # first tens of MCS run up to 6% slower).
+    # The graphs of the warm-up and the timed run first, then the
+    # pre-relaxation (part of building the workload, whatever --warmup says)
+    # and the warm-up back to back with the timed run: those two skip the
+    # coordinate read-back, so the GPU runs without a gap of more than a host
+    # synchronization up to the timed window (after an idle gap of a few ms
+    # its first tens of MCS run up to 6% slower).
     u.prepareGraphs(args.steps)
+    if args.warmup > 0:
+        u.prepareGraphs(args.warmup)
     u.setReadBack(False)
     if args.prerelax > 0:
         st, _ = u.run(args.prerelax)
