@@ -159,7 +159,7 @@ def cpu_single_core(u, box, bonds, colors, seed, target_seconds: float) -> dict:
     return out
 
 
-def network_section(seed: int, mcs: int, kind: int, c_l: float, shape=None) -> dict:
+def network_section(seed: int, mcs: int, kind: int, c_l: float, shape=None, clear_pass: int = 0) -> dict:
     """Star network: 12800 4-arm stars of arm length 29 (1,497,600 monomers)
     with arm ends crosslinked to other stars, on a 512x256x256 periodic box
     (occupied fraction 0.357; the same monomer count on 256^3 would be 0.72,
@@ -175,6 +175,8 @@ def network_section(seed: int, mcs: int, kind: int, c_l: float, shape=None) -> d
     n = xyz.shape[0]
     log(f"network: {n} monomers, {u.nColors()} species {u.colorSizes().tolist()}, "
         f"built in {time.perf_counter() - tic:.1f} s")
+    if clear_pass:
+        u.setClearPass(clear_pass)
     if shape:
         block, slots = shape
         trials = {}
@@ -361,6 +363,8 @@ def main() -> int:
     ap.add_argument("--list-group", type=int, default=0,
                     help="pass-2 threads per pass-1 list (0: default; -1: one list per workgroup)")
     ap.add_argument("--list-depth", type=int, default=0, help="pass-2 entries per thread in flight (0: default)")
+    ap.add_argument("--clear-pass", type=int, default=0, choices=[0, 1, 2],
+                    help="pass that clears the other parity's proposals (0: default)")
     ap.add_argument("--profile-mcs", type=int, default=20)
     ap.add_argument("--cpu-seconds", type=float, default=20.0,
                     help="target seconds of the all-cores oracle sample (its calibration run overestimates, "
@@ -421,6 +425,8 @@ def main() -> int:
         u.setGraphChunk(args.graph_chunk)
     if args.list_group or args.list_depth:
         u.setPerformWalk(P.LEGACY_WALK if args.list_group < 0 else args.list_group, args.list_depth)
+    if args.clear_pass:
+        u.setClearPass(args.clear_pass)
     colors = u.colors(xyz.shape[0])
     n = int(xyz.shape[0])
     log(f"initialize done in {time.perf_counter() - tic:.1f} s; species sizes {u.colorSizes().tolist()}; "
@@ -613,7 +619,7 @@ def main() -> int:
     u.cleanup()
     if not args.no_network and args.network_mcs > 0:
         nshape = tuple(int(v) for v in args.network_shape.split("x")) if args.network_shape else None
-        out["network"] = network_section(args.seed, args.network_mcs, kind, c_l, nshape)
+        out["network"] = network_section(args.seed, args.network_mcs, kind, c_l, nshape, args.clear_pass)
     if not args.no_batch and args.replicas > 0:
         out["batched"] = batched_vs_plain(args.replicas, args.replica_mcs, args.seed, kind, args.tune_sample,
                                           args.tune_mcs, plain=not args.no_plain,

@@ -151,6 +151,11 @@ int pscbfm_set_perform_block(pscbfm_updater* h, uint32_t block);
  * threads_per_list = 0 or 0xFFFFFFFF (the default) walks one list per
  * workgroup.  Results do not depend on it. */
 int pscbfm_set_perform_walk(pscbfm_updater* h, uint32_t threads_per_list, uint32_t entries_per_thread);
+/* The pass of each substep (1 or 2; 0 = automatic: pass 1 on the bit
+ * lattice with at most two bonds per monomer, else pass 2) that zeroes the
+ * other parity's proposal buffer, the reference's separate clear
+ * (CRITICAL.cu:561-573).  Results do not depend on it. */
+int pscbfm_set_clear_pass(pscbfm_updater* h, uint32_t pass);
 /* Substeps per captured graph (0 = automatic, at most 4096:
  * PSCBFM_INVALID_ARGUMENT above). */
 int pscbfm_set_graph_chunk(pscbfm_updater* h, uint32_t substeps);

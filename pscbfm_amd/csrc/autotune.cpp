@@ -2,6 +2,7 @@
 #include "autotune.h"
 
 #include <algorithm>
+#include <cmath>
 #include <map>
 #include <mutex>
 #include <stdexcept>
@@ -61,6 +62,8 @@ double time_system(const SystemLayout& L, uint32_t kind, uint64_t seed, LaunchSh
     DeviceSystem dev;
     dev.allocate(L, kind, seed, /*withLists=*/true, uint32_t(max_grid(L.maxColorCount)), st);
     s.grid = grid_for(L.maxColorCount, s.block * s.slots);
+    s.performBlock = automatic_perform_block(L, s.block);
+    s.clearPass = automatic_clear_pass(L, kind);
     const uint32_t nc = std::max<uint32_t>(1u, uint32_t(L.colors.size()));
     const uint32_t g = nc;  // one sweep per replay
     hipGraph_t graph = nullptr;
@@ -100,6 +103,32 @@ TuneResult pick(std::vector<TuneTrial> trials) {
 }
 
 }  // namespace
+
+uint32_t automatic_perform_block(const SystemLayout& L, uint32_t block) {
+    // Pass 2 walks one compacted pass-1 list per workgroup.  In a dense
+    // system (at least a quarter of the cells occupied) a list holds about a
+    // fifth of its capacity, so 128 threads have about one entry each: on
+    // the 512^3 melt at 0.5 that sweeps 1.0% faster than 256 threads (1.9%
+    // when pass 2 also clears the proposals) and 64 threads 3.2% slower.
+    // Dilute systems have fuller lists and keep the pass-1 block (the
+    // replica mix at 0.1-0.5 runs pass 2 16% faster with 256 than with 128).
+    const uint32_t log2Cells = L.codec.log2L[0] + L.codec.log2L[1] + L.codec.log2L[2];
+    const double occupied = 8.0 * double(L.nMonomers) / std::ldexp(1.0, int(log2Cells));
+    return occupied >= 0.25 && block > 128u ? 128u : 0u;
+}
+
+uint32_t automatic_clear_pass(const SystemLayout& L, uint32_t latticeKind) {
+    // The clear (16 bytes per thread and step) goes into pass 1 where that
+    // pass has issue slots to spare: the bit lattice with at most two bonds
+    // per monomer, whose pass 1 gathers the fewest words.  Measured, pass 1
+    // against pass 2: +1.2% sweeps on the 512^3 melt at 0.5, +1.9% at 0.25,
+    // +0.4% on 256^3; the 4-arm star network loses 2.3-3.4% and the uint8
+    // lattice 0.6-2% (its pass 1 moves eight times the plane bytes), so
+    // those clear in pass 2.
+    uint32_t maxDegree = 0;
+    for (const ColorInfo& ci : L.colors) maxDegree = std::max(maxDegree, ci.maxDegree);
+    return latticeKind == kLatticeBit && maxDegree <= 2u ? 1u : 2u;
+}
 
 const std::vector<LaunchShape>& tune_candidates() {
     static const std::vector<LaunchShape> c = [] {

@@ -25,6 +25,14 @@ struct TuneResult {
     bool cached = false;
 };
 
+// The choices a single system's shape makes by itself once its pass-1
+// block is set (the updater's defaults, also applied to every candidate
+// the autotuner times):
+// pass-2 threads per workgroup, 0 = the pass-1 block;
+uint32_t automatic_perform_block(const SystemLayout& L, uint32_t block);
+// the pass (1 or 2) that zeroes the other parity's proposal buffer.
+uint32_t automatic_clear_pass(const SystemLayout& L, uint32_t latticeKind);
+
 // Candidate shapes: block in {64, 128, 192, 256} x slots in {1, 2, 4}.
 const std::vector<LaunchShape>& tune_candidates();
 

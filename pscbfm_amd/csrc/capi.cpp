@@ -297,6 +297,10 @@ int pscbfm_set_perform_walk(pscbfm_updater* h, uint32_t threads_per_list, uint32
     return guarded(h, [&](pscbfm::Updater& u) { u.setPerformWalk(threads_per_list, entries_per_thread); });
 }
 
+int pscbfm_set_clear_pass(pscbfm_updater* h, uint32_t pass) {
+    return guarded(h, [&](pscbfm::Updater& u) { u.setClearPass(pass); });
+}
+
 int pscbfm_set_read_back(pscbfm_updater* h, int on) {
     return guarded(h, [&](pscbfm::Updater& u) { u.setReadBack(on != 0); });
 }

@@ -21,6 +21,9 @@ struct LaunchShape {
     // entries (1 or 4) in flight per thread.
     uint32_t listGroup = 0;
     uint32_t listDepth = 4;
+    // The pass (1 or 2) of a single system's substep that zeroes the other
+    // parity's proposal buffer.  Batches clear in pass 2.
+    uint32_t clearPass = 2;
     uint32_t passTwoBlock() const { return performBlock ? performBlock : block; }
     uint32_t listStride() const { return slots * block; }  // list entries per workgroup
     // Pass-1 lists each XCD produces: workgroup w runs on XCD w % 8 and

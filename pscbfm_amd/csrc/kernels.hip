@@ -2,7 +2,7 @@
 //
 // Pass 1 (check):   CRITICAL.cu:298-374  kernelSimulationScBFMCheckSpecies
 // Pass 2 (perform): CRITICAL.cu:376-411  kernelSimulationScBFMPerformSpeciesAndApply
-// Proposal clear:   CRITICAL.cu:561-573  (here folded into pass 2, other parity)
+// Proposal clear:   CRITICAL.cu:561-573  (here folded into pass 1 or 2, other parity)
 //
 // Design (wave64, CDNA4):
 //  * positions are Z-curve codes, so the own lattice cell is the code itself
@@ -204,12 +204,32 @@ __device__ __forceinline__ uint32_t lane_rank(uint64_t mask) {
 // List entry: x = species-local index | direction << 29, y = old code.
 constexpr uint32_t kDirShift = 29;
 
-// Pass 1 of substep t for logical workgroup lb of one system.  The
+// Zeroes slice lb of nb of the proposal buffer of the other parity, 16 bytes
+// per thread and step: pass 2 of substep t-1 was its last reader and pass 1
+// of substep t+1 is its next writer, so either pass of substep t may do it
+// (LaunchShape::clearPass).  In pass 1 the stores go out ahead of its loads
+// and stream while its waves wait on them and compute the bond test; in
+// pass 2 they overlap the list walk.
+__device__ __forceinline__ void clear_other_parity(const SysDesc& d, uint64_t t, uint32_t lb, uint32_t nb) {
+    const gptr<uint64_t> prev = (gptr<uint64_t>)(d.proposal.global() + size_t((t + 1u) & 1u) * d.latWords);
+    const uint32_t quads = d.latWords >> 2;
+    const uint32_t per = (quads + nb - 1) / nb;
+    const uint32_t q0 = lb * per;
+    const uint32_t q1 = min(q0 + per, quads);
+    for (uint32_t q = q0 + threadIdx.x; q < q1; q += blockDim.x) {
+        prev[2 * size_t(q)] = 0u;
+        prev[2 * size_t(q) + 1] = 0u;
+    }
+}
+
+// Pass 1 of substep t for logical workgroup lb of nb of one system.  The
 // survivors go to `list` (capacity SLOTS * blockDim), their number to *count.
+// With `clear` it also zeroes slice lb of nb of the other parity's proposals.
 template <int LAT, int SLOTS, int MAXDEG>
-__device__ __forceinline__ void check_body(const SysDesc& d, uint64_t t, SubstepDraw draw, uint32_t lb,
-                                           gptr<uint64_t> list, gptr<uint32_t> count, uint32_t* sForbidden,
-                                           uint32_t* sCount) {
+__device__ __forceinline__ void check_body(const SysDesc& d, uint64_t t, SubstepDraw draw, uint32_t lb, uint32_t nb,
+                                           bool clear, gptr<uint64_t> list, gptr<uint32_t> count,
+                                           uint32_t* sForbidden, uint32_t* sCount) {
+    if (clear) clear_other_parity(d, t, lb, nb);
     const ColorInfo ci = load_color(d.colors.global(), draw.species);
     const gptr<uint32_t> proposal = d.proposal.global();
 
@@ -357,25 +377,11 @@ __device__ __forceinline__ void apply_move(const SysDesc& d, gptr<uint32_t> pos,
     }
 }
 
-// Zeroes slice lb of nb of the proposal buffer of the other parity, 16 bytes
-// per thread and step: pass 2 of substep t-1 was its last reader and pass 1
-// of substep t+1 is its next writer.
-__device__ __forceinline__ void clear_other_parity(const SysDesc& d, uint64_t t, uint32_t lb, uint32_t nb) {
-    const gptr<uint64_t> prev = (gptr<uint64_t>)(d.proposal.global() + size_t((t + 1u) & 1u) * d.latWords);
-    const uint32_t quads = d.latWords >> 2;
-    const uint32_t per = (quads + nb - 1) / nb;
-    const uint32_t q0 = lb * per;
-    const uint32_t q1 = min(q0 + per, quads);
-    for (uint32_t q = q0 + threadIdx.x; q < q1; q += blockDim.x) {
-        prev[2 * size_t(q)] = 0u;
-        prev[2 * size_t(q) + 1] = 0u;
-    }
-}
-
 // Pass 2 of substep t over one pass-1 list.  The accepted count is added to
-// *accSlot, an entry only this workgroup writes.
+// *accSlot, an entry only this workgroup writes.  With `clear` it also zeroes
+// slice lb of nb of the other parity's proposals.
 template <int LAT>
-__device__ __forceinline__ void perform_body(const SysDesc& d, const Launch& L, uint32_t lb, uint32_t nb,
+__device__ __forceinline__ void perform_body(const SysDesc& d, const Launch& L, uint32_t lb, uint32_t nb, bool clear,
                                              gptr<const uint64_t> list, uint32_t count, gptr<uint64_t> accSlot,
                                              uint32_t* sSum) {
     const uint64_t t = L.t;
@@ -383,9 +389,8 @@ __device__ __forceinline__ void perform_body(const SysDesc& d, const Launch& L, 
     const ColorInfo ci = load_color(d.colors.global(), draw.species);
     const uint32_t B = blockDim.x;
 
-    // The other parity's proposal clear, one slice per workgroup: streaming
-    // these stores here overlaps them with the latency-bound list walk.
-    clear_other_parity(d, t, lb, nb);
+    if (clear) clear_other_parity(d, t, lb, nb);
+
     const AxisMasks masks = axis_masks(d);
     const gptr<const uint64_t> T64 =
         (gptr<const uint64_t>)(d.proposal.global() + size_t(t & 1u) * d.latWords);
@@ -418,25 +423,26 @@ __device__ __forceinline__ void perform_body(const SysDesc& d, const Launch& L, 
 
 // ---- single system ----------------------------------------------------------------
 template <int LAT, int SLOTS, int MAXDEG>
-__global__ __launch_bounds__(256) void k_check(SysDesc d, uint32_t k) {
+__global__ __launch_bounds__(256) void k_check(SysDesc d, uint32_t k, uint32_t clear) {
     __shared__ uint32_t sForbidden[16];
     __shared__ uint32_t sCount;
     Launch L;
     if (!begin_substep(d, k, L)) return;
     const uint32_t lb = xcd_block();
-    check_body<LAT, SLOTS, MAXDEG>(d, L.t, L.draw, lb,
+    check_body<LAT, SLOTS, MAXDEG>(d, L.t, L.draw, lb, gridDim.x, clear != 0u,
                                    (gptr<uint64_t>)d.list.global() + size_t(lb) * (SLOTS * blockDim.x),
                                    d.listCount.global() + lb, sForbidden, &sCount);
 }
 
 template <int LAT>
-__global__ __launch_bounds__(256) void k_perform(SysDesc d, uint32_t k, uint32_t listStride) {
+__global__ __launch_bounds__(256) void k_perform(SysDesc d, uint32_t k, uint32_t listStride, uint32_t clear) {
     __shared__ uint32_t sSum[4];
     Launch L;
     if (!begin_substep(d, k, L)) return;
     const uint32_t lb = xcd_block();  // same numbering as pass 1: list lb
-    perform_body<LAT>(d, L, lb, gridDim.x, (gptr<const uint64_t>)d.list.global() + size_t(lb) * listStride,
-                      d.listCount.global()[lb], L.acceptedByBlock + blockIdx.x, sSum);
+    perform_body<LAT>(d, L, lb, gridDim.x, clear != 0u,
+                      (gptr<const uint64_t>)d.list.global() + size_t(lb) * listStride, d.listCount.global()[lb],
+                      L.acceptedByBlock + blockIdx.x, sSum);
 }
 
 // Pass 2 of a single system with several lists per workgroup: each group
@@ -448,7 +454,7 @@ __global__ __launch_bounds__(256) void k_perform(SysDesc d, uint32_t k, uint32_t
 // the state with the list, then the proposal words of DEPTH moves at once.
 template <int LAT, int DEPTH>
 __global__ __launch_bounds__(256) void k_perform_lists(SysDesc d, uint32_t k, uint32_t listStride,
-                                                       uint32_t listsPerXcd, uint32_t group) {
+                                                       uint32_t listsPerXcd, uint32_t group, uint32_t clear) {
     __shared__ uint32_t sSum[4];
     const uint32_t xcd = blockIdx.x & 7u, local = blockIdx.x >> 3;
     const uint32_t gi = threadIdx.x / group, lane = threadIdx.x - gi * group;
@@ -466,7 +472,7 @@ __global__ __launch_bounds__(256) void k_perform_lists(SysDesc d, uint32_t k, ui
     const uint64_t t = L.t;
     const uint32_t slotOffset =
         ((gptr<const uint32_t>)(d.colors.global() + L.draw.species))[offsetof(ColorInfo, slotOffset) / 4];
-    clear_other_parity(d, t, blockIdx.x, gridDim.x);
+    if (clear) clear_other_parity(d, t, blockIdx.x, gridDim.x);
     const AxisMasks masks = axis_masks(d);
     const gptr<const uint64_t> T64 =
         (gptr<const uint64_t>)(d.proposal.global() + size_t(t & 1u) * d.latWords);
@@ -524,7 +530,7 @@ __global__ __launch_bounds__(256) void k_check_batch(const SysDesc* __restrict__
     const SysDesc& d = descs[tile.replica];
     Launch L;
     if (!begin_substep(d, k, L)) return;
-    check_body<LAT, SLOTS, MAXDEG>(d, L.t, L.draw, tile.chunk,
+    check_body<LAT, SLOTS, MAXDEG>(d, L.t, L.draw, tile.chunk, tile.chunks, false,
                                    global_ptr(reinterpret_cast<uint64_t*>(lists)) + size_t(blockIdx.x) * (SLOTS * blockDim.x),
                                    global_ptr(listCounts) + blockIdx.x, sForbidden, &sCount);
 }
@@ -539,7 +545,7 @@ __global__ __launch_bounds__(256) void k_perform_batch(const SysDesc* __restrict
     const SysDesc& d = descs[tile.replica];
     Launch L;
     if (!begin_substep(d, k, L)) return;
-    perform_body<LAT>(d, L, tile.chunk, tile.chunks,
+    perform_body<LAT>(d, L, tile.chunk, tile.chunks, true,
                       global_ptr(reinterpret_cast<const uint64_t*>(lists)) + size_t(blockIdx.x) * listStride,
                       global_ptr(listCounts)[blockIdx.x], L.acceptedByBlock + tile.chunk, sSum);
 }
@@ -637,7 +643,8 @@ void check_launch(const SysDesc& d, const BatchLaunch* b, uint32_t k, const Laun
         hipLaunchKernelGGL((k_check_batch<LAT, SLOTS, MAXDEG>), dim3(b->tiles), blk, 0, st, b->descs, b->tileTable,
                            b->lists, b->listCounts, k);
     else
-        hipLaunchKernelGGL((k_check<LAT, SLOTS, MAXDEG>), dim3(s.grid), blk, 0, st, d, k);
+        hipLaunchKernelGGL((k_check<LAT, SLOTS, MAXDEG>), dim3(s.grid), blk, 0, st, d, k,
+                           uint32_t(s.clearPass == 1u));
 }
 
 template <int LAT, int SLOTS>
@@ -667,13 +674,14 @@ void perform_launch(const SysDesc& d, const BatchLaunch* b, uint32_t k, const La
         hipLaunchKernelGGL((k_perform_batch<LAT>), dim3(b->tiles), blk, 0, st, b->descs, b->tileTable, b->lists,
                            b->listCounts, k, s.listStride());
     else if (s.listGroup == 0)
-        hipLaunchKernelGGL((k_perform<LAT>), dim3(s.grid), blk, 0, st, d, k, s.listStride());
+        hipLaunchKernelGGL((k_perform<LAT>), dim3(s.grid), blk, 0, st, d, k, s.listStride(),
+                           uint32_t(s.clearPass != 1u));
     else if (s.listDepth >= 4)
         hipLaunchKernelGGL((k_perform_lists<LAT, 4>), dim3(s.passTwoGrid()), blk, 0, st, d, k, s.listStride(),
-                           s.listsPerXcd(), s.listGroup);
+                           s.listsPerXcd(), s.listGroup, uint32_t(s.clearPass != 1u));
     else
         hipLaunchKernelGGL((k_perform_lists<LAT, 1>), dim3(s.passTwoGrid()), blk, 0, st, d, k, s.listStride(),
-                           s.listsPerXcd(), s.listGroup);
+                           s.listsPerXcd(), s.listGroup, uint32_t(s.clearPass != 1u));
 }
 
 }  // namespace

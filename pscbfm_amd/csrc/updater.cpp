@@ -9,6 +9,7 @@
 #include <sstream>
 #include <stdexcept>
 
+#include "autotune.h"
 #include "noalloc.h"
 
 namespace pscbfm {
@@ -235,6 +236,12 @@ void Updater::setPerformWalk(uint32_t group, uint32_t depth) {
     if (initialized_) applyLaunchShape();
 }
 
+void Updater::setClearPass(uint32_t pass) {
+    if (pass > 2) throw std::invalid_argument("[setClearPass] pass must be 0 (automatic), 1 or 2");
+    clearPassOverride_ = pass;
+    if (initialized_) applyLaunchShape();
+}
+
 void Updater::applyLaunchShape() {
     shape_.block = blockOverride_ ? blockOverride_ : 256;
     shape_.slots = slotsOverride_ ? slotsOverride_ : 2;
@@ -247,17 +254,7 @@ void Updater::applyLaunchShape() {
         shape_.listGroup = walkGroupOverride_;
         shape_.listDepth = walkDepthOverride_ ? walkDepthOverride_ : 4;
     }
-    // Pass 2 walks one compacted pass-1 list per workgroup and clears one
-    // slice of the other-parity proposal lattice.  In a dense system (at
-    // least a quarter of the cells occupied) the lists hold about a fifth of
-    // their capacity and the clear is small per monomer, so 128 threads do
-    // it: 1.9% faster sweeps than 256 on the 512^3 melt at 0.5.  Dilute
-    // systems have fuller lists and a larger clear per monomer and keep the
-    // pass-1 block (the replica mix at 0.1-0.5 runs pass 2 16% faster with
-    // 256 than with 128).
-    const double occupied = cells_ ? 8.0 * double(n_) / double(cells_) : 0.0;
-    shape_.performBlock = performOverride_ ? performOverride_
-                                           : (occupied >= 0.25 && shape_.block > 128u ? 128u : 0u);
+    shape_.performBlock = performOverride_ ? performOverride_ : automatic_perform_block(layout_, shape_.block);
     if (shape_.listGroup) {
         // several lists per workgroup: whole lists, at most 256 threads
         if (!performOverride_) shape_.performBlock = 256;
@@ -265,6 +262,7 @@ void Updater::applyLaunchShape() {
         shape_.performBlock = shape_.passTwoBlock() / shape_.listGroup * shape_.listGroup;
     }
     shape_.grid = grid_for(layout_.maxColorCount, shape_.block * shape_.slots);
+    shape_.clearPass = clearPassOverride_ ? clearPassOverride_ : automatic_clear_pass(layout_, latticeKind_);
     performGridMax_ = std::max(performGridMax_, shape_.passTwoGrid());
     dropGraphs();
 }

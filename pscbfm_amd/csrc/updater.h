@@ -100,6 +100,10 @@ public:
     // kLegacyWalk (the default) walks one list per workgroup.
     static constexpr uint32_t kLegacyWalk = 0xFFFFFFFFu;
     void setPerformWalk(uint32_t group, uint32_t depth);
+    // The pass (1 or 2) that zeroes the other parity's proposal buffer each
+    // substep; 0 = automatic (pass 1 on the bit lattice with at most two
+    // bonds per monomer, else pass 2).
+    void setClearPass(uint32_t pass);
     // R250 mode: from the next run on, every substep draws its species
     // (r250 % nColors) and kernel seed (r250) from this generator in the
     // reference's order, instead of the counter draw of (seed, substep).
@@ -210,6 +214,7 @@ private:
     uint32_t graphChunk_ = 0;   // 0: choose automatically
     uint32_t blockOverride_ = 0, slotsOverride_ = 0, performOverride_ = 0;
     uint32_t walkGroupOverride_ = 0, walkDepthOverride_ = 0;
+    uint32_t clearPassOverride_ = 0;
 
     // device
     bool initialized_ = false;

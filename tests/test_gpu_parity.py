@@ -7,7 +7,7 @@ import numpy as np
 import pytest
 
 import pscbfm_amd as P
-from helpers import chains, gpu_vs_oracle, make_updater, ordered_chains, stars, Sys
+from helpers import chains, gpu_vs_oracle, make_updater, oracle_for, ordered_chains, stars, Sys
 
 pytestmark = pytest.mark.gpu
 
@@ -82,23 +82,50 @@ def test_invariants_after_run(kind):
     u.cleanup()
 
 
+@pytest.mark.parametrize("clear_pass", [1, 2])
 @pytest.mark.parametrize("kind", [P.LATTICE_BIT, P.LATTICE_U8])
-def test_proposal_lattice_after_run(kind):
+def test_proposal_lattice_after_run(kind, clear_pass):
     # The reference clears its proposal lattice before every substep
     # (CRITICAL.cu:561-573) and its legacy updater checks that it is empty
-    # after a run.  Here substep t marks buffer t & 1 and its pass 2 clears
-    # the other buffer, so after substeps 0..T-1 the buffer the next substep
-    # uses is empty and the other holds the last substep's marks: at least one
-    # per accepted move (their targets are distinct), at most one per member
-    # of the drawn species.
+    # after a run.  Here substep t marks buffer t & 1 and its pass 1 or pass 2
+    # clears the other buffer, so after substeps 0..T-1 the buffer the next
+    # substep uses is empty and the other holds the last substep's marks: at
+    # least one per accepted move (their targets are distinct), at most one
+    # per member of the drawn species.
     s = chains((64, 64, 64), 50, 50, seed=4)
     u = make_updater(s, seed=1, kind=kind)
+    u.setClearPass(clear_pass)
     _, log = u.run(25, log=True)
     steps = len(log)
     bits = u.proposalBits()
     assert bits[steps & 1] == 0
     assert 0 < log[-1] <= bits[(steps - 1) & 1] <= int(max(u.colorSizes()))
     u.cleanup()
+
+
+@pytest.mark.parametrize("walk", [0, 128])
+@pytest.mark.parametrize("kind", [P.LATTICE_BIT, P.LATTICE_U8])
+def test_clear_pass_does_not_change_results(kind, walk):
+    # Either pass of substep t may zero the other parity's proposal buffer:
+    # pass 2 of substep t-1 was its last reader and pass 1 of substep t+1 is
+    # its next writer.  Switching the pass between runs keeps that order, so
+    # runs that alternate equal the oracle's uninterrupted run.
+    s = chains((64, 64, 64), 40, 64, seed=13)
+    u = make_updater(s, 9, kind)
+    if walk:
+        u.setPerformWalk(walk, 1)
+    logs = []
+    for clear_pass, m in [(1, 4), (2, 3), (1, 5), (0, 3)]:
+        u.setClearPass(clear_pass)
+        _, log = u.run(m, log=True)
+        logs.append(log)
+    x = u.getPositions(s.n)
+    colors = u.colors(s.n)
+    u.cleanup()
+    o = oracle_for(s, colors, 9)
+    _, olog = o.run(0, 15 * (int(colors.max()) + 1))
+    assert np.array_equal(np.concatenate(logs), olog)
+    assert np.array_equal(x, o.xyz)
 
 
 def test_single_monomer_always_moves():

@@ -515,6 +515,15 @@ def test_perform_walk_is_validated():
             u.setPerformWalk(group, depth)
 
 
+def test_clear_pass_is_validated():
+    u = P.UpdaterGPUScBFM_AB_Type()
+    for p in (0, 1, 2):
+        u.setClearPass(p)
+    for p in (3, 7):
+        with pytest.raises(P.InvalidArgument):
+            u.setClearPass(p)
+
+
 def test_device_code_has_no_flat_memory_instructions(tmp_path):
     # Every device buffer is reached through a global-address-space pointer
     # (descriptor pointers are DeviceBuffer, readable on the device only as
