@@ -220,8 +220,9 @@ int pscbfm_download_lattice(const pscbfm_updater* h, uint8_t* cells, uint64_t n_
  * them have no 16-bit partner code, so that the uint8-lattice pass 1 reads the
  * full neighbour matrix for them.  Staged or initialized handle; no device work. */
 int pscbfm_count_partner_escapes(const pscbfm_updater* h, uint64_t* partners, uint64_t* escapes);
-/* Number of set bits in both proposal lattices (0 after every complete substep pair
- * once the next pass 1 has cleared the older buffer). */
+/* Number of set bits in both proposal lattices.  After substeps 0..T-1 the
+ * buffer of parity T & 1, which substep T marks next, is empty, and the other
+ * holds the marks of substep T-1. */
 int pscbfm_count_proposal_bits(const pscbfm_updater* h, uint64_t* bits_even, uint64_t* bits_odd);
 
 /* Host helpers shared with the kernels (no device work). */
