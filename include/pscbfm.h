@@ -331,9 +331,9 @@ int pscbfm_batch_info(pscbfm_batch* b, uint32_t id, pscbfm_replica_info* info);
 /* Unfolded coordinates of replica `id` in its input order (3*n ints). */
 int pscbfm_batch_get_positions(pscbfm_batch* b, uint32_t id, int32_t* xyz, uint32_t n);
 /* pscbfm_batch_get_positions for `count` replicas at once: xyz[k] receives
- * 3*n[k] ints of replica ids[k].  One synchronization for all of them, the
- * decodes on a few host threads.  No counterpart in the reference (it reads
- * one system back per run, CRITICAL.cu:686-695). */
+ * 3*n[k] ints of replica ids[k].  The coordinates are decoded on the device
+ * and copied back in rounds of up to 64 MiB.  No counterpart in the
+ * reference (it reads one system back per run, CRITICAL.cu:686-695). */
 int pscbfm_batch_get_positions_many(pscbfm_batch* b, const uint32_t* ids, uint32_t count, int32_t* const* xyz,
                                     const uint32_t* n);
 /* Autotuner for a batch: times `mcs` sweeps of a scratch batch of the staged

@@ -579,7 +579,8 @@ class Batch:
         return out
 
     def positions_many(self, rids, ns) -> list:
-        """positions() of several replicas with one synchronization."""
+        """positions() of several replicas: decoded on the device, copied back
+        in rounds of up to 64 MiB."""
         ids = np.ascontiguousarray(rids, dtype=np.uint32)
         counts = np.ascontiguousarray(ns, dtype=np.uint32)
         if ids.shape != counts.shape:

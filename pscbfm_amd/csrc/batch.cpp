@@ -51,6 +51,8 @@ Batch::Batch(int gpu, uint32_t latticeKind, uint32_t tileCapacity, uint32_t repl
     hip_check(hipMalloc(&dDescs_, sizeof(SysDesc) * replicaCap_), "hipMalloc descs");
     hip_check(hipMalloc(&dStops_, sizeof(uint64_t) * replicaCap_), "hipMalloc stops");
     hip_check(hipMalloc(&dGather_, sizeof(uint64_t) * 3 * replicaCap_), "hipMalloc gather");
+    jobsCap_ = std::max<uint32_t>(replicaCap_, 64u);
+    hip_check(hipMalloc(&dJobs_, sizeof(UnfoldJob) * jobsCap_), "hipMalloc unfold jobs");
     descs_.assign(replicaCap_, SysDesc{});
     replicas_.resize(replicaCap_);
     freeSlots_.assign(8, {});
@@ -91,8 +93,10 @@ Batch::~Batch() {
     dropGraphs();
     for (auto& r : replicas_) r.dev.reset();
     for (void* p : {static_cast<void*>(dDescs_), static_cast<void*>(dTiles_), static_cast<void*>(dLists_),
-                    static_cast<void*>(dListCounts_), static_cast<void*>(dStops_), static_cast<void*>(dGather_)})
+                    static_cast<void*>(dListCounts_), static_cast<void*>(dStops_), static_cast<void*>(dGather_),
+                    static_cast<void*>(dJobs_), static_cast<void*>(dXyz_)})
         if (p) (void)hipFree(p);
+    if (hXyz_) (void)hipHostFree(hXyz_);
     if (stream_) (void)hipStreamDestroy(stream_);
 }
 
@@ -101,6 +105,21 @@ void Batch::setPerformBlock(uint32_t block) {
     hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
     shape_.performBlock = block ? block : default_perform_block(shape_.block);
     dropGraphs();
+}
+
+void Batch::reserveReadBack(uint64_t monomers) {
+    const uint64_t want = std::min(monomers, kReadBackRecords);
+    if (want <= xyzCap_) return;
+    const uint64_t cap = std::min(kReadBackRecords, std::max(want, xyzCap_ + xyzCap_ / 2));
+    hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+    if (dXyz_) (void)hipFree(dXyz_);
+    if (hXyz_) (void)hipHostFree(hXyz_);
+    dXyz_ = nullptr;
+    hXyz_ = nullptr;
+    xyzCap_ = 0;
+    hip_check(hipMalloc(&dXyz_, cap * 12), "hipMalloc read-back");
+    hip_check(hipHostMalloc(reinterpret_cast<void**>(&hXyz_), cap * 12, hipHostMallocDefault), "hipHostMalloc read-back");
+    xyzCap_ = cap;
 }
 
 uint32_t Batch::tilesNeeded(const SystemLayout& L) const {
@@ -170,6 +189,10 @@ uint32_t Batch::add(const SystemLayout& L, uint64_t seed, uint64_t startMcs) {
     r.info.nMonomers = L.nMonomers;
     r.info.tiles = nT;
     r.info.deviceBytes = r.dev->bytes;
+    uint64_t liveMonomers = 0;
+    for (const Replica& q : replicas_)
+        if (q.live) liveMonomers += q.layout.nMonomers;
+    reserveReadBack(liveMonomers);
     const uint32_t bucket = degree_bucket(r.dev->desc.maxDegree);
     if (bucket > graphDegree_) {
         graphDegree_ = bucket;
@@ -353,61 +376,78 @@ const BatchReplicaInfo& Batch::info(uint32_t id) const {
     return replicas_[id].info;
 }
 
-void Batch::positions(uint32_t id, int32_t* xyz, uint32_t n) {
-    if (id >= replicaCap_ || !replicas_[id].live) throw std::invalid_argument("batch: no such replica");
-    const Replica& r = replicas_[id];
-    if (n != r.layout.nMonomers) throw std::invalid_argument("batch: count differs from the replica's monomers");
-    hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
-    r.dev->positions(r.layout, xyz, stream_);
-}
+void Batch::positions(uint32_t id, int32_t* xyz, uint32_t n) { positionsMany(&id, 1, &xyz, &n); }
 
 void Batch::positionsMany(const uint32_t* ids, uint32_t count, int32_t* const* xyz, const uint32_t* n) {
-    std::vector<uint64_t> offset(size_t(count) + 1, 0);
+    uint64_t total = 0;
     for (uint32_t k = 0; k < count; ++k) {
         const uint32_t id = ids[k];
         if (id >= replicaCap_ || !replicas_[id].live) throw std::invalid_argument("batch: no such replica");
-        const Replica& r = replicas_[id];
-        if (n[k] != r.layout.nMonomers) throw std::invalid_argument("batch: count differs from the replica's monomers");
+        if (n[k] != replicas_[id].layout.nMonomers)
+            throw std::invalid_argument("batch: count differs from the replica's monomers");
         if (!xyz[k] && n[k] != 0) throw std::invalid_argument("batch: null output");
-        offset[k + 1] = offset[k] + (r.dev->positionsBytes(r.layout) + 255u) / 256u * 256u;
+        total += n[k];
     }
-    if (count == 0) return;
-    std::unique_ptr<char[]> host(new char[offset[count]]);  // no zero fill: the copies write it all
+    if (total == 0) return;
+    hip_check(hipSetDevice(gpu_), "hipSetDevice");
+    reserveReadBack(total);  // already held for the live replicas, except for repeated ids
     hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+
+    // Rounds of at most xyzCap_ records and jobsCap_ ranges: decode on the
+    // device, one copy to pinned memory, then each range to its caller.
+    struct Range {
+        int32_t* dst;
+        uint64_t out;
+        uint32_t count;
+    };
+    std::vector<UnfoldJob> jobs;
+    std::vector<Range> ranges;
+    uint64_t used = 0;
+    uint32_t blocks = 0;
+    auto flush = [&] {
+        if (jobs.empty()) return;
+        hip_check(hipMemcpyAsync(dJobs_, jobs.data(), jobs.size() * sizeof(UnfoldJob), hipMemcpyHostToDevice, stream_),
+                  "unfold jobs H2D");
+        hip_check(launch_unfold_jobs(dJobs_, uint32_t(jobs.size()), blocks, dXyz_, stream_), "launch unfold");
+        hip_check(hipMemcpyAsync(hXyz_, dXyz_, used * 12, hipMemcpyDeviceToHost, stream_), "positions D2H");
+        hip_check(hipStreamSynchronize(stream_), "sync");
+        auto copyOut = [&](uint32_t part, uint32_t parts) {
+            for (size_t q = part; q < ranges.size(); q += parts)
+                std::memcpy(ranges[q].dst, hXyz_ + 3 * ranges[q].out, uint64_t(ranges[q].count) * 12);
+        };
+        // a few threads for large rounds (a host copy runs at a few GB/s per thread)
+        const uint32_t parts = used * 12 >= (uint64_t(8) << 20) ? std::min<uint32_t>(8u, uint32_t(ranges.size())) : 1u;
+        std::vector<std::thread> pool;
+        for (uint32_t p = 1; p < parts; ++p) {
+            try {
+                pool.emplace_back(copyOut, p, parts);
+            } catch (...) {
+                for (size_t q = p; q < ranges.size(); q += parts)  // this part on the calling thread
+                    std::memcpy(ranges[q].dst, hXyz_ + 3 * ranges[q].out, uint64_t(ranges[q].count) * 12);
+            }
+        }
+        copyOut(0, parts);
+        for (std::thread& th : pool) th.join();
+        jobs.clear();
+        ranges.clear();
+        used = 0;
+        blocks = 0;
+    };
     for (uint32_t k = 0; k < count; ++k) {
         const Replica& r = replicas_[ids[k]];
-        hip_check(hipMemcpyAsync(host.get() + offset[k], r.dev->positionsBase(), r.dev->positionsBytes(r.layout),
-                                 hipMemcpyDeviceToHost, stream_),
-                  "positions D2H");
-    }
-    hip_check(hipStreamSynchronize(stream_), "sync");
-    const uint32_t nThreads = std::min<uint32_t>(count, 8u);
-    std::atomic<uint32_t> next{0};
-    std::mutex failMutex;
-    std::exception_ptr failure;
-    auto work = [&] {
-        try {
-            for (uint32_t k = next++; k < count; k = next++) {
-                const Replica& r = replicas_[ids[k]];
-                r.dev->decodePositions(r.layout, host.get() + offset[k], xyz[k]);
-            }
-        } catch (...) {
-            std::lock_guard<std::mutex> lock(failMutex);
-            if (!failure) failure = std::current_exception();
-            next = count;
-        }
-    };
-    std::vector<std::thread> pool;
-    for (uint32_t t = 1; t < nThreads; ++t) {
-        try {
-            pool.emplace_back(work);
-        } catch (...) {
-            break;  // fewer decoding threads
+        for (uint32_t first = 0; first < n[k];) {
+            if (used == xyzCap_ || jobs.size() == jobsCap_) flush();
+            const uint32_t take = uint32_t(std::min<uint64_t>(n[k] - first, xyzCap_ - used));
+            UnfoldJob j = r.dev->unfoldJob(first, take, used);
+            j.firstBlock = blocks;
+            jobs.push_back(j);
+            ranges.push_back(Range{xyz[k] + 3 * size_t(first), used, take});
+            used += take;
+            blocks += (take + kUnfoldBlock - 1) / kUnfoldBlock;
+            first += take;
         }
     }
-    work();
-    for (std::thread& th : pool) th.join();
-    if (failure) std::rethrow_exception(failure);
+    flush();
 }
 
 uint32_t Batch::liveReplicas() const {
@@ -419,7 +459,7 @@ uint32_t Batch::liveReplicas() const {
 uint64_t Batch::deviceBytes() const {
     uint64_t b = sizeof(SysDesc) * uint64_t(replicaCap_) + sizeof(Tile) * uint64_t(tileCap_) +
                  sizeof(uint2) * uint64_t(tileCap_) * shape_.block * shape_.slots + 4ull * tileCap_ +
-                 32ull * replicaCap_;
+                 32ull * replicaCap_ + sizeof(UnfoldJob) * uint64_t(jobsCap_) + 12 * xyzCap_;
     for (const auto& r : replicas_)
         if (r.live) b += r.dev->bytes;
     return b;

@@ -64,8 +64,10 @@ public:
 
     const BatchReplicaInfo& info(uint32_t id) const;
     void positions(uint32_t id, int32_t* xyz, uint32_t n);
-    // positions() of `count` replicas with one synchronization: every copy is
-    // queued first, then the decodes run on a few host threads.
+    // positions() of `count` replicas: one launch decodes the coordinates of
+    // as many of them as the read-back buffer holds on the device, one copy
+    // brings them into pinned host memory, and a few host threads hand each
+    // replica its range; larger requests take several such rounds.
     void positionsMany(const uint32_t* ids, uint32_t count, int32_t* const* xyz, const uint32_t* n);
     uint32_t liveReplicas() const;
     uint32_t usedTiles() const { return usedTiles_; }
@@ -103,6 +105,8 @@ private:
     void enqueue(const GraphPlan& plan, uint64_t substeps);
     void dropGraphs();
     void growTiles(uint32_t needed);
+    // Read-back buffers for `monomers` records (at most kReadBackRecords).
+    void reserveReadBack(uint64_t monomers);
 
     int gpu_;
     uint32_t latticeKind_;
@@ -135,6 +139,15 @@ private:
     // graphs cover the remainder.
     static constexpr uint32_t kGraphSubsteps = 64;
     uint64_t launched_ = 0;
+    // Coordinate read-back: decoded records on the device, their pinned host
+    // copy and the job table, sized to the live replicas up to 64 MiB of
+    // records (a request beyond that runs in rounds).
+    static constexpr uint64_t kReadBackRecords = (uint64_t(64) << 20) / 12;
+    int32_t* dXyz_ = nullptr;
+    int32_t* hXyz_ = nullptr;
+    uint64_t xyzCap_ = 0;  // records
+    UnfoldJob* dJobs_ = nullptr;
+    uint32_t jobsCap_ = 0;
 };
 
 }  // namespace pscbfm

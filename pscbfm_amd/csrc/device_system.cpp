@@ -14,7 +14,7 @@
 //   acc      uint64 per workgroup accepted-move running totals
 //   ids      16-bit partner codes, two per word (uint8 lattice only)
 //   rowBase  uint32 per (species, row): base slot of the codes (uint8 lattice only)
-//   slotOf   uint32 per monomer   slot of each monomer (only with withLists)
+//   slotOf   uint32 per monomer   slot of each monomer (coordinate read-back)
 //   xyz      3 int32 per monomer  unfolded coordinates decoded for read-back (only with withLists)
 #include "device_system.h"
 
@@ -70,9 +70,10 @@ Plan plan(const SystemLayout& L, uint32_t latticeKind, bool withLists, uint32_t 
     const bool codes = latticeKind == kLatticeU8;
     p.bytes[P_IDS] = codes ? std::max<uint64_t>(L.ids.size(), 1) * 4 : 4;
     p.bytes[P_ROWB] = codes ? std::max<uint64_t>(L.rowBase.size(), 1) * 4 : 4;
-    // a system with its own launches decodes its coordinates on the device
+    // coordinates are decoded on the device; a system with its own launches
+    // also holds their output buffer (a batch shares one)
     const uint64_t nMono = std::max<uint64_t>(L.nMonomers, 1);
-    p.bytes[P_SLOTOF] = withLists ? nMono * 4 : 4;
+    p.bytes[P_SLOTOF] = nMono * 4;
     p.bytes[P_XYZ] = withLists ? nMono * 12 : 4;
     p.total = 0;
     for (int i = 0; i < P_N; ++i) {
@@ -124,7 +125,7 @@ void DeviceSystem::allocate(const SystemLayout& L, uint32_t latticeKind, uint64_
     desc.latticeKind = latticeKind;
     desc.maxColorCount = L.maxColorCount;
     desc.accEntries = this->accEntries;
-    slotOfDev = withLists ? reinterpret_cast<uint32_t*>(base + p.off[P_SLOTOF]) : nullptr;
+    slotOfDev = reinterpret_cast<uint32_t*>(base + p.off[P_SLOTOF]);
     xyzDev = withLists ? reinterpret_cast<int32_t*>(base + p.off[P_XYZ]) : nullptr;
 
     hip_check(hipMemsetAsync(arena, 0, p.total, stream), "hipMemsetAsync");
@@ -132,7 +133,7 @@ void DeviceSystem::allocate(const SystemLayout& L, uint32_t latticeKind, uint64_
         if (n) hip_check(hipMemcpyAsync(base + p.off[piece], src, n, hipMemcpyHostToDevice, stream), "H2D");
     };
     up(P_POS, L.pos.data(), uint64_t(L.nSlots) * 4);
-    if (withLists) up(P_SLOTOF, L.slotOf.data(), uint64_t(L.nMonomers) * 4);
+    up(P_SLOTOF, L.slotOf.data(), uint64_t(L.nMonomers) * 4);
     up(P_IMGX, L.img[0].data(), uint64_t(L.nSlots) * 4);
     up(P_IMGY, L.img[1].data(), uint64_t(L.nSlots) * 4);
     up(P_IMGZ, L.img[2].data(), uint64_t(L.nSlots) * 4);
@@ -200,48 +201,29 @@ uint64_t DeviceSystem::acceptedTotal(uint32_t entries, hipStream_t stream) const
 }
 
 void DeviceSystem::positions(const SystemLayout& L, int32_t* xyzOut, hipStream_t stream) const {
-    const uint32_t ns = L.nSlots;
-    if (!ns) return;
-    if (xyzDev) {
-        // decoded on the device in the caller's order: 12 bytes per monomer
-        // cross the bus and the host only receives them
-        if (!L.nMonomers) return;
-        hip_check(launch_unfold(desc, slotOfDev, L.nMonomers, xyzDev, stream), "launch unfold");
-        hip_check(hipMemcpyAsync(xyzOut, xyzDev, uint64_t(L.nMonomers) * 12, hipMemcpyDeviceToHost, stream),
-                  "positions D2H");
-        hip_check(hipStreamSynchronize(stream), "sync");
-        return;
-    }
-    // pos, imgX, imgY and imgZ are consecutive pieces of the arena: one copy
-    const char* first = reinterpret_cast<const char*>(static_cast<uint32_t*>(desc.pos));
-    const uint64_t span = uint64_t(reinterpret_cast<const char*>(static_cast<int32_t*>(desc.imgZ)) - first) + uint64_t(ns) * 4;
-    // Staging: a per-thread buffer reused across calls up to 64 MiB (no page
-    // faults inside the copy for the many small systems of a batch); larger
-    // systems take a one-off buffer without zero fill.
-    static thread_local std::vector<uint32_t> reused;
-    std::unique_ptr<uint32_t[]> oneOff;
-    const uint64_t words = (span + 3) / 4;
-    uint32_t* host;
-    if (span <= (uint64_t(64) << 20)) {
-        if (reused.size() < words) reused.resize(words);
-        host = reused.data();
-    } else {
-        oneOff.reset(new uint32_t[words]);
-        host = oneOff.get();
-    }
-    hip_check(hipMemcpyAsync(host, first, span, hipMemcpyDeviceToHost, stream), "positions D2H");
+    if (!L.nMonomers) return;
+    if (!xyzDev) throw std::logic_error("positions: this system decodes into its batch's buffer");
+    // decoded on the device in the caller's order: 12 bytes per monomer
+    // cross the bus and the host only receives them
+    hip_check(launch_unfold(desc, slotOfDev, L.nMonomers, xyzDev, stream), "launch unfold");
+    hip_check(hipMemcpyAsync(xyzOut, xyzDev, uint64_t(L.nMonomers) * 12, hipMemcpyDeviceToHost, stream),
+              "positions D2H");
     hip_check(hipStreamSynchronize(stream), "sync");
-    decodePositions(L, host, xyzOut);
 }
 
-void DeviceSystem::decodePositions(const SystemLayout& L, const void* copy, int32_t* xyzOut) const {
-    const char* first = reinterpret_cast<const char*>(static_cast<uint32_t*>(desc.pos));
-    const char* h = static_cast<const char*>(copy);
-    auto at = [&](const void* dev) { return h + (reinterpret_cast<const char*>(dev) - first); };
-    const int32_t* img[3] = {reinterpret_cast<const int32_t*>(at(desc.imgX)),
-                             reinterpret_cast<const int32_t*>(at(desc.imgY)),
-                             reinterpret_cast<const int32_t*>(at(desc.imgZ))};
-    L.unfolded(reinterpret_cast<const uint32_t*>(h), img, xyzOut);
+UnfoldJob DeviceSystem::unfoldJob(uint32_t first, uint32_t count, uint64_t out) const {
+    UnfoldJob j{};
+    j.pos = desc.pos;
+    j.img[0] = desc.imgX;
+    j.img[1] = desc.imgY;
+    j.img[2] = desc.imgZ;
+    j.slotOf = slotOfDev;
+    for (int a = 0; a < 3; ++a) j.mask[a] = desc.mask[a];
+    j.first = first;
+    j.count = count;
+    j.firstBlock = 0;
+    j.out = out;
+    return j;
 }
 
 }  // namespace pscbfm

@@ -9,6 +9,7 @@
 #include <vector>
 
 #include "common.h"
+#include "kernels.h"
 #include "layout.h"
 
 namespace pscbfm {
@@ -24,9 +25,9 @@ struct DeviceSystem {
     uint64_t cells = 0;
     uint32_t accEntries = 0;  // entries of DevState::acceptedByBlock
     SysDesc desc{};
-    // Systems with their own launches (withLists): the slot of every monomer
-    // and a buffer of unfolded coordinates in monomer order, so positions()
-    // decodes on the device.
+    // The slot of every monomer, for decoding coordinates on the device, and
+    // (systems with their own launches, withLists) the buffer positions()
+    // decodes them into.  Batch replicas decode into the batch's buffer.
     uint32_t* slotOfDev = nullptr;
     int32_t* xyzDev = nullptr;
 
@@ -43,17 +44,11 @@ struct DeviceSystem {
     DevState readState(hipStream_t stream) const;
     void writeState(const DevState& s, hipStream_t stream);
     uint64_t acceptedTotal(uint32_t entries, hipStream_t stream) const;
-    // Unfolded coordinates in the caller's monomer order.
+    // Unfolded coordinates in the caller's monomer order (withLists systems).
     void positions(const SystemLayout& L, int32_t* xyzOut, hipStream_t stream) const;
-    // The pieces positions() copies: pos, imgX, imgY and imgZ are consecutive
-    // in the arena, so one copy of positionsBytes() bytes from positionsBase()
-    // holds them; decodePositions() turns such a copy into coordinates.
-    const void* positionsBase() const { return static_cast<uint32_t*>(desc.pos); }
-    uint64_t positionsBytes(const SystemLayout& L) const {
-        return uint64_t(reinterpret_cast<const char*>(static_cast<int32_t*>(desc.imgZ)) - reinterpret_cast<const char*>(static_cast<uint32_t*>(desc.pos))) +
-               uint64_t(L.nSlots) * 4;
-    }
-    void decodePositions(const SystemLayout& L, const void* copy, int32_t* xyzOut) const;
+    // The job that decodes monomers [first, first + count) into records
+    // out.. of a shared buffer (launch_unfold_jobs; firstBlock left 0).
+    UnfoldJob unfoldJob(uint32_t first, uint32_t count, uint64_t out) const;
 };
 
 // Workgroups covering the largest species with `perGroup` monomers each.

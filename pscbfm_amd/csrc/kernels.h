@@ -65,6 +65,22 @@ hipError_t launch_gather_state(const SysDesc* descs, uint32_t nDescs, uint64_t* 
 // xyz[3i..3i+2] = unfolded corner of monomer i (slot slotOf[i]): the
 // Z-code's axis bits plus image counter times box edge, for i < n.
 hipError_t launch_unfold(const SysDesc& d, const uint32_t* slotOf, uint32_t n, int32_t* xyz, hipStream_t st);
+// One range of one system's monomers for launch_unfold_jobs.  Pointers are
+// device addresses; ranges of one launch are in order of firstBlock, the
+// first at 0, each taking ceil(count / kUnfoldBlock) workgroups.
+constexpr uint32_t kUnfoldBlock = 256;
+struct UnfoldJob {
+    const uint32_t* pos;     // Z-curve code per slot
+    const int32_t* img[3];   // image counters per slot
+    const uint32_t* slotOf;  // slot of each monomer
+    uint32_t mask[3];        // Z-curve axis masks
+    uint32_t first;          // first monomer of the range
+    uint32_t count;          // monomers in the range
+    uint32_t firstBlock;     // first workgroup of the range
+    uint64_t out;            // record (3 int32) of the range's first monomer in xyz
+};
+// xyz[3(out+i)..] = unfolded corner of monomer first+i of every job, i < count.
+hipError_t launch_unfold_jobs(const UnfoldJob* jobs, uint32_t nJobs, uint32_t blocks, int32_t* xyz, hipStream_t st);
 // Fills n 32-bit words with v (graph-safe replacement of a memset).
 hipError_t launch_fill_u32(uint32_t* p, uint64_t n, uint32_t v, hipStream_t st);
 // Streaming copy of n16 16-byte elements (bandwidth and counter reference).

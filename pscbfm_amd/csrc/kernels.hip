@@ -608,18 +608,46 @@ __device__ __forceinline__ uint32_t compact_bits(uint32_t m, uint32_t mask) {
     return v;
 }
 
+// Unfolded corner of the monomer in slot s: the code's axis bits plus the
+// image counter times the box edge, written to out[0..2].
+__device__ __forceinline__ void unfold_slot(gptr<const uint32_t> pos, const gptr<const int32_t> img[3],
+                                            const uint32_t mask[3], uint32_t s, gptr<int32_t> out) {
+    const uint32_t m = pos[s];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const int32_t edge = int32_t(1u << __builtin_popcount(mask[a]));
+        out[a] = int32_t(compact_bits(m, mask[a])) + img[a][s] * edge;
+    }
+}
+
 __global__ void k_unfold(SysDesc d, const uint32_t* __restrict__ slotOf, uint32_t n, int32_t* __restrict__ xyz) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const uint32_t s = slotOf[i];
-    const uint32_t m = d.pos.global()[s];
     const gptr<const int32_t> img[3] = {d.imgX.global(), d.imgY.global(), d.imgZ.global()};
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-        const uint32_t mask = d.mask[a];
-        const int32_t edge = int32_t(1u << __builtin_popcount(mask));
-        xyz[3 * size_t(i) + a] = int32_t(compact_bits(m, mask)) + img[a][s] * edge;
+    const uint32_t mask[3] = {d.mask[0], d.mask[1], d.mask[2]};
+    unfold_slot(d.pos.global(), img, mask, global_ptr(slotOf)[i], global_ptr(xyz) + 3 * size_t(i));
+}
+
+// Several monomer ranges of several systems in one launch: workgroup b
+// serves the last job whose firstBlock is at most b (jobs in order of
+// firstBlock, the first at 0), monomers first + (b - firstBlock) * blockDim
+// + threadIdx of that job, written as records out + ... of xyz.
+__global__ __launch_bounds__(256) void k_unfold_jobs(const UnfoldJob* __restrict__ jobs, uint32_t nJobs,
+                                                     int32_t* __restrict__ xyz) {
+    const uint32_t b = blockIdx.x;
+    uint32_t lo = 0, hi = nJobs;
+    while (hi - lo > 1u) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (jobs[mid].firstBlock <= b) lo = mid;
+        else hi = mid;
     }
+    const UnfoldJob& J = jobs[lo];
+    const uint32_t i = (b - J.firstBlock) * blockDim.x + threadIdx.x;
+    if (i >= J.count) return;
+    const gptr<const int32_t> img[3] = {global_ptr(J.img[0]), global_ptr(J.img[1]), global_ptr(J.img[2])};
+    const uint32_t mask[3] = {J.mask[0], J.mask[1], J.mask[2]};
+    unfold_slot(global_ptr(J.pos), img, mask, global_ptr(J.slotOf)[J.first + i],
+                global_ptr(xyz) + 3 * (J.out + i));
 }
 
 __global__ void k_fill_u32(uint32_t* p, uint64_t n, uint32_t v) {
@@ -746,6 +774,12 @@ hipError_t launch_gather_state(const SysDesc* descs, uint32_t nDescs, uint64_t* 
 hipError_t launch_unfold(const SysDesc& d, const uint32_t* slotOf, uint32_t n, int32_t* xyz, hipStream_t st) {
     if (n == 0) return hipSuccess;
     hipLaunchKernelGGL(k_unfold, dim3((n + 255) / 256), dim3(256), 0, st, d, slotOf, n, xyz);
+    return hipGetLastError();
+}
+
+hipError_t launch_unfold_jobs(const UnfoldJob* jobs, uint32_t nJobs, uint32_t blocks, int32_t* xyz, hipStream_t st) {
+    if (nJobs == 0 || blocks == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_unfold_jobs, dim3(blocks), dim3(kUnfoldBlock), 0, st, jobs, nJobs, xyz);
     return hipGetLastError();
 }
 

@@ -26,7 +26,8 @@ KERNELS = {"k_check_batch": "pass 1, batch (k_check_batch)", "k_perform_batch": 
            "k_advance_batch": "counter advance, batch", "k_gather_state": "state gather, batch",
            "k_set_stops": "stop points, batch", "k_check": "pass 1 (k_check)", "k_perform": "pass 2 (k_perform)",
            "k_copy_u4": "copy reference", "k_advance": "counter advance", "k_fill_u32": "fill",
-           "k_unfold": "coordinate decode (read-back)", "k_perform_lists": "pass 2, several lists (k_perform_lists)"}
+           "k_unfold": "coordinate decode (read-back)", "k_perform_lists": "pass 2, several lists (k_perform_lists)",
+           "k_unfold_jobs": "coordinate decode, batch (read-back)"}
 
 
 _NAME = re.compile(r"\b(k_[a-z0-9_]+?)(?=[<(]|$)")

@@ -141,6 +141,26 @@ def test_large_replica_spread_over_all_xcds():
     assert b.info(small).live
 
 
+def test_read_back_in_rounds_equals_alone():
+    # The batch decodes coordinates on the device into a buffer of at most
+    # 64 MiB of records (5,592,405 monomers) and copies them back in rounds:
+    # the 8.4M-monomer melt spans two rounds, so its ranges are split, and a
+    # request that repeats it starts a range in the middle of a round.
+    melt, mseed = ordered_chains((512, 512, 512), 32768, 256), 42
+    small, sseed = chains((32, 32, 32), 20, 16, seed=7), 3
+    print("read-back rounds", melt.n, small.n, flush=True)
+    b = P.Batch(tile_capacity=0, replica_capacity=4)
+    rs = b.add(stage_updater(small, sseed))
+    rm = b.add(stage_updater(melt, mseed))
+    b.run(2)
+    want_m = _alone(melt, mseed, [2])[0]
+    want_s = _alone(small, sseed, [2])[0]
+    got = b.positions_many([rs, rm, rs, rm], [small.n, melt.n, small.n, melt.n])
+    assert np.array_equal(got[0], want_s) and np.array_equal(got[2], want_s)
+    assert np.array_equal(got[1], want_m) and np.array_equal(got[3], want_m)
+    assert np.array_equal(b.positions(rm, melt.n), want_m)
+
+
 def test_capacity_and_id_errors():
     s = chains((32, 32, 32), 20, 16, seed=7)
     b = P.Batch(tile_capacity=8, replica_capacity=2)
