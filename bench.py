@@ -50,9 +50,9 @@ def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def make_updater(box, xyz, bonds, seed, kind, block=0, slots=0):
+def make_updater(box, xyz, bonds, seed, kind, block=0, slots=0, spatial_order=True):
     sd = W.SystemData(tuple(box), xyz, bonds, seed, "ordered")
-    return W.stage(sd, kind, block=block, slots=slots, initialize=True)
+    return W.stage(sd, kind, block=block, slots=slots, initialize=True, spatial_order=spatial_order)
 
 
 def algorithmic_bytes(stats, lattice_bytes_per_cell: float):
@@ -159,7 +159,8 @@ def cpu_single_core(u, box, bonds, colors, seed, target_seconds: float) -> dict:
     return out
 
 
-def network_section(seed: int, mcs: int, kind: int, c_l: float, shape=None, clear_pass: int = 0) -> dict:
+def network_section(seed: int, mcs: int, kind: int, c_l: float, shape=None, clear_pass: int = 0,
+                    spatial_order: bool = True) -> dict:
     """Star network: 12800 4-arm stars of arm length 29 (1,497,600 monomers)
     with arm ends crosslinked to other stars, on a 512x256x256 periodic box
     (occupied fraction 0.357; the same monomer count on 256^3 would be 0.72,
@@ -171,7 +172,7 @@ def network_section(seed: int, mcs: int, kind: int, c_l: float, shape=None, clea
     tic = time.perf_counter()
     xyz, bonds = P.gen_star_network(box, 12800, 4, 29, seed)
     sd = W.SystemData(box, xyz, bonds, seed, "saw")
-    u = W.stage(sd, kind, initialize=True)
+    u = W.stage(sd, kind, initialize=True, spatial_order=spatial_order)
     n = xyz.shape[0]
     log(f"network: {n} monomers, {u.nColors()} species {u.colorSizes().tolist()}, "
         f"built in {time.perf_counter() - tic:.1f} s")
@@ -363,6 +364,8 @@ def main() -> int:
     ap.add_argument("--list-group", type=int, default=0,
                     help="pass-2 threads per pass-1 list (0: default; -1: one list per workgroup)")
     ap.add_argument("--list-depth", type=int, default=0, help="pass-2 entries per thread in flight (0: default)")
+    ap.add_argument("--slot-order", choices=["spatial", "reference"], default="spatial",
+                    help="slot order inside each species (results are the same)")
     ap.add_argument("--clear-pass", type=int, default=0, choices=[0, 1, 2],
                     help="pass that clears the other parity's proposals (0: default)")
     ap.add_argument("--profile-mcs", type=int, default=20)
@@ -418,7 +421,8 @@ def main() -> int:
     tic = time.perf_counter()
     xyz = P.gen_linear_chains_ordered(box, n_chains, args.chain)
     bonds = P.chain_bonds(n_chains, args.chain)
-    u = make_updater(box, xyz, bonds, args.seed, kind, args.block, args.slots)
+    u = make_updater(box, xyz, bonds, args.seed, kind, args.block, args.slots,
+                     spatial_order=args.slot_order == "spatial")
     if args.perform_block:
         u.setPerformBlock(args.perform_block)
     if args.graph_chunk:
@@ -573,7 +577,8 @@ def main() -> int:
         other = "u8" if args.lattice == "bit" else "bit"
         okind = P.LATTICE_U8 if other == "u8" else P.LATTICE_BIT
         oc_l = 1.0 if other == "u8" else 0.125
-        v = make_updater(box, xyz, bonds, args.seed, okind, args.block, args.slots)
+        v = make_updater(box, xyz, bonds, args.seed, okind, args.block, args.slots,
+                         spatial_order=args.slot_order == "spatial")
         same_mcs = args.prerelax + args.warmup + args.steps
         v.run(same_mcs)
         cross = bool(np.array_equal(v.getPositions(n), timed_xyz))
@@ -619,7 +624,8 @@ def main() -> int:
     u.cleanup()
     if not args.no_network and args.network_mcs > 0:
         nshape = tuple(int(v) for v in args.network_shape.split("x")) if args.network_shape else None
-        out["network"] = network_section(args.seed, args.network_mcs, kind, c_l, nshape, args.clear_pass)
+        out["network"] = network_section(args.seed, args.network_mcs, kind, c_l, nshape, args.clear_pass,
+                                         args.slot_order == "spatial")
     if not args.no_batch and args.replicas > 0:
         out["batched"] = batched_vs_plain(args.replicas, args.replica_mcs, args.seed, kind, args.tune_sample,
                                           args.tune_mcs, plain=not args.no_plain,

@@ -138,6 +138,7 @@ _SIGNATURES = {
     "pscbfm_read_positions_back": (C.c_int, [_H]),
     "pscbfm_set_perform_walk": (C.c_int, [_H, C.c_uint32, C.c_uint32]),
     "pscbfm_set_clear_pass": (C.c_int, [_H, C.c_uint32]),
+    "pscbfm_set_slot_order": (C.c_int, [_H, C.c_int]),
     "pscbfm_prepare_graphs": (C.c_int, [_H, C.c_int32]),
     "pscbfm_run_ex": (C.c_int, [_H, C.c_int32, _u32p, C.c_int, C.c_int, C.POINTER(RunStats)]),
     "pscbfm_last_run_stats": (C.c_int, [_H, C.POINTER(RunStats)]),
@@ -375,6 +376,11 @@ class UpdaterGPUScBFM_AB_Type:
         """Pass-2 walk: threads per pass-1 list (0 or LEGACY_WALK, the default:
         one list per workgroup) and entries in flight per thread (1 or 4; 0: 4)."""
         self._call("pscbfm_set_perform_walk", int(threads_per_list), int(entries_per_thread))
+
+    def setSlotOrder(self, spatial: bool = True) -> None:
+        """Slot order inside each species (before initialize): spatial, by
+        starting Z-curve code (default), or the reference's monomer order."""
+        self._call("pscbfm_set_slot_order", int(bool(spatial)))
 
     def setClearPass(self, pass_: int = 0) -> None:
         """The pass (1 or 2; 0: automatic, pass 1 on the bit lattice with at

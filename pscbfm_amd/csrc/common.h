@@ -268,6 +268,7 @@ struct SysDesc {
     DeviceBuffer<uint32_t> listCount;     // entries of each list
     DeviceBuffer<const uint32_t> ids;     // compressed partner ids (see layout.h)
     DeviceBuffer<const uint32_t> rowBase; // kMaxConnectivity per species: base slot of each row's deltas
+    DeviceBuffer<const uint32_t> key;     // per slot: inner move hash of the member's reference index
     uint64_t seed;           // key of the substep draw
     uint32_t mask[3];        // Z-curve bit masks of x, y, z
     uint32_t nColors;

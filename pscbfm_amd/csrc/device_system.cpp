@@ -16,6 +16,7 @@
 //   rowBase  uint32 per (species, row): base slot of the codes (uint8 lattice only)
 //   slotOf   uint32 per monomer   slot of each monomer (coordinate read-back)
 //   xyz      3 int32 per monomer  unfolded coordinates decoded for read-back (only with withLists)
+//   key      uint32 per slot      inner move hash of the member's reference index
 #include "device_system.h"
 
 #include <algorithm>
@@ -40,7 +41,7 @@ namespace {
 uint64_t align_up(uint64_t v, uint64_t a) { return (v + a - 1) / a * a; }
 
 enum { P_POS, P_IMGX, P_IMGY, P_IMGZ, P_LIST, P_LCOUNT, P_NBR, P_LAT, P_PROP, P_COLORS, P_BOND, P_STATE, P_ACC, P_IDS, P_ROWB,
-       P_SLOTOF, P_XYZ, P_N };
+       P_SLOTOF, P_XYZ, P_KEY, P_N };
 
 struct Plan {
     uint64_t bytes[P_N];
@@ -75,6 +76,7 @@ Plan plan(const SystemLayout& L, uint32_t latticeKind, bool withLists, uint32_t 
     const uint64_t nMono = std::max<uint64_t>(L.nMonomers, 1);
     p.bytes[P_SLOTOF] = nMono * 4;
     p.bytes[P_XYZ] = withLists ? nMono * 12 : 4;
+    p.bytes[P_KEY] = nSlots * 4;
     p.total = 0;
     for (int i = 0; i < P_N; ++i) {
         p.off[i] = p.total;
@@ -114,6 +116,7 @@ void DeviceSystem::allocate(const SystemLayout& L, uint32_t latticeKind, uint64_
     desc.bondMask = reinterpret_cast<const uint32_t*>(base + p.off[P_BOND]);
     desc.ids = reinterpret_cast<const uint32_t*>(base + p.off[P_IDS]);
     desc.rowBase = reinterpret_cast<const uint32_t*>(base + p.off[P_ROWB]);
+    desc.key = reinterpret_cast<const uint32_t*>(base + p.off[P_KEY]);
     desc.nSlots = std::max<uint32_t>(L.nSlots, 1u);
     desc.state = reinterpret_cast<DevState*>(base + p.off[P_STATE]);
     desc.seed = seed;
@@ -133,6 +136,7 @@ void DeviceSystem::allocate(const SystemLayout& L, uint32_t latticeKind, uint64_
         if (n) hip_check(hipMemcpyAsync(base + p.off[piece], src, n, hipMemcpyHostToDevice, stream), "H2D");
     };
     up(P_POS, L.pos.data(), uint64_t(L.nSlots) * 4);
+    up(P_KEY, L.key.data(), uint64_t(L.nSlots) * 4);
     up(P_SLOTOF, L.slotOf.data(), uint64_t(L.nMonomers) * 4);
     up(P_IMGX, L.img[0].data(), uint64_t(L.nSlots) * 4);
     up(P_IMGY, L.img[1].data(), uint64_t(L.nSlots) * 4);

@@ -242,6 +242,9 @@ __device__ __forceinline__ void check_body(const SysDesc& d, uint64_t t, Substep
     const gptr<const uint32_t> posAll = (gptr<const uint32_t>)d.pos.global();
     const gptr<const uint32_t> pos = posAll + ci.slotOffset;
     const gptr<const uint32_t> nbr = d.nbr.global();
+    // Inner hash of each member's species-local index in the reference's
+    // order, the key of its draw (slots may be in spatial order).
+    const gptr<const uint32_t> keys = d.key.global() + ci.slotOffset;
     // Partner ids: the bit-lattice kernel reads the full matrix rows; the
     // uint8-lattice kernel, whose plane words cost more traffic, reads the
     // 16-bit codes (half the bytes) and the full matrix only for escapes.
@@ -258,7 +261,7 @@ __device__ __forceinline__ void check_body(const SysDesc& d, uint64_t t, Substep
     // the value, so every load of a thread sits in one basic block and all of
     // them are in flight together.
     // First batch of loads: own codes and partner ids.
-    uint32_t j[SLOTS], m[SLOTS], partner[SLOTS][MAXDEG];
+    uint32_t j[SLOTS], m[SLOTS], hj[SLOTS], partner[SLOTS][MAXDEG];
     bool live[SLOTS];
 #pragma unroll
     for (int s = 0; s < SLOTS; ++s) {
@@ -266,6 +269,7 @@ __device__ __forceinline__ void check_body(const SysDesc& d, uint64_t t, Substep
         live[s] = j[s] < ci.count;
         const uint32_t jj = live[s] ? j[s] : 0u;  // count > 0 for every species
         m[s] = pos[jj];
+        hj[s] = keys[jj];
         if constexpr (LAT == kLatticeU8) {
 #pragma unroll
             for (int q = 0; q < MAXDEG / 2; ++q) {
@@ -294,11 +298,11 @@ __device__ __forceinline__ void check_body(const SysDesc& d, uint64_t t, Substep
         }
     }
     // Second batch: partner codes and the front plane.  CRITICAL.cu:322-323: the
-    // draw is keyed by the species-local index.
+    // draw is keyed by the species-local index in the reference's order.
     uint32_t dir[SLOTS], to[SLOTS], partnerCode[SLOTS][MAXDEG], occ[SLOTS];
 #pragma unroll
     for (int s = 0; s < SLOTS; ++s) {
-        dir[s] = move_hash(move_hash(j[s]) ^ draw.seed) % 6u;
+        dir[s] = move_hash(hj[s] ^ draw.seed) % 6u;
         const Move mv = make_move(m[s], dir[s], masks);
         to[s] = mv.to;
 #pragma unroll

@@ -179,16 +179,69 @@ SystemLayout build_layout(const SystemInput& in, const std::vector<uint8_t>* col
     }
     L.nSlots = slot;
 
-    // Stable placement by original index, as iToiNew in the reference.
+    // Wrapped starting corners as Z-curve codes with their image counters.
+    const ZTables z(L.codec);
+    std::vector<uint32_t> code(n);
+    std::vector<int32_t> image(3 * size_t(n));
+    parallel_for(n, [&](uint32_t lo, uint32_t hi) {
+        for (uint32_t i = lo; i < hi; ++i) {
+            uint32_t w[3];
+            for (int a = 0; a < 3; ++a) {
+                const int32_t v = in.xyz[3 * size_t(i) + a];
+                const int32_t im = floor_div_pow2(v, L.codec.log2L[a]);
+                image[3 * size_t(i) + a] = im;
+                w[a] = uint32_t(v - im * int32_t(in.box[a]));
+            }
+            code[i] = z.encode(w[0], w[1], w[2]);
+        }
+    });
+    // Members of each species in the reference's order, stable by original
+    // index as iToiNew (UpdaterGPUScBFM_AB_Type.cu:601-607): a member's rank
+    // there is the species-local index that keys its draws.
+    std::vector<std::vector<uint32_t>> members(nColors);
+    for (uint32_t c = 0; c < nColors; ++c) members[c].reserve(count[c]);
+    for (uint32_t i = 0; i < n; ++i) members[L.color[i]].push_back(i);
+    // Slots: the members in that order, or sorted by starting code when the
+    // order is spatial (codes are distinct: no two corners share a cell).
+    // Each slot keeps the inner hash of its member's reference rank.
+    std::vector<std::vector<uint32_t>> slots(nColors);
+    auto placeSpecies = [&](uint32_t c) {
+        std::vector<uint32_t>& order = slots[c];
+        order.resize(members[c].size());
+        for (uint32_t r = 0; r < order.size(); ++r) order[r] = r;
+        if (in.spatialOrder) {
+            std::vector<uint64_t> packed(order.size());
+            for (uint32_t r = 0; r < order.size(); ++r) packed[r] = (uint64_t(code[members[c][r]]) << 32) | r;
+            std::sort(packed.begin(), packed.end());
+            for (uint32_t r = 0; r < order.size(); ++r) order[r] = uint32_t(packed[r]);
+        }
+    };
+    {
+        std::vector<std::thread> pool;
+        const bool threads = n >= (1u << 16);
+        for (uint32_t c = 1; c < nColors; ++c) {
+            if (threads) {
+                try {
+                    pool.emplace_back(placeSpecies, c);
+                    continue;
+                } catch (...) {
+                }
+            }
+            placeSpecies(c);
+        }
+        placeSpecies(0);
+        for (std::thread& t : pool) t.join();
+    }
     L.slotOf.assign(n, 0);
     L.monomerOf.assign(L.nSlots, 0xFFFFFFFFu);
-    std::vector<uint32_t> next(nColors);
-    for (uint32_t c = 0; c < nColors; ++c) next[c] = L.colors[c].slotOffset;
-    for (uint32_t i = 0; i < n; ++i) {
-        const uint32_t s = next[L.color[i]]++;
-        L.slotOf[i] = s;
-        L.monomerOf[s] = i;
-    }
+    L.key.assign(L.nSlots, 0);
+    for (uint32_t c = 0; c < nColors; ++c)
+        for (uint32_t p = 0; p < slots[c].size(); ++p) {
+            const uint32_t r = slots[c][p], i = members[c][r], s = L.colors[c].slotOffset + p;
+            L.slotOf[i] = s;
+            L.monomerOf[s] = i;
+            L.key[s] = move_hash(r);
+        }
 
     // Neighbour matrices: row r of species c holds the r-th bond partner of
     // each member, so a wavefront reads one row contiguously.
@@ -206,18 +259,11 @@ SystemLayout build_layout(const SystemInput& in, const std::vector<uint8_t>* col
     // unfolded coordinate can be rebuilt.
     L.pos.assign(L.nSlots, 0);
     for (int a = 0; a < 3; ++a) L.img[a].assign(L.nSlots, 0);
-    const ZTables z(L.codec);
     parallel_for(n, [&](uint32_t lo, uint32_t hi) {
         for (uint32_t i = lo; i < hi; ++i) {
             const uint32_t s = L.slotOf[i];
-            uint32_t w[3];
-            for (int a = 0; a < 3; ++a) {
-                const int32_t v = in.xyz[3 * size_t(i) + a];
-                const int32_t im = floor_div_pow2(v, L.codec.log2L[a]);
-                L.img[a][s] = im;
-                w[a] = uint32_t(v - im * int32_t(in.box[a]));
-            }
-            L.pos[s] = z.encode(w[0], w[1], w[2]);
+            L.pos[s] = code[i];
+            for (int a = 0; a < 3; ++a) L.img[a][s] = image[3 * size_t(i) + a];
         }
     });
 

@@ -15,6 +15,12 @@ struct SystemInput {
     std::vector<int32_t> xyz;        // unfolded corner, 3 per monomer
     Adjacency bonds;                 // directed bond lists as registered
     uint8_t forbidden[512];          // reference bond table, index (dx&7)|(dy&7)<<3|(dz&7)<<6
+    // Slot order inside each species: false = by monomer index (the
+    // reference's order); true = by the Z-curve code of the starting corner,
+    // so that a wave's members, their partners and their lattice words lie
+    // close together.  Results do not depend on it: a move's draw is keyed by
+    // the member's rank in the reference's order either way.
+    bool spatialOrder = true;
 };
 
 struct SystemLayout {
@@ -26,6 +32,9 @@ struct SystemLayout {
     std::vector<uint32_t> slotOf;        // monomer -> slot
     std::vector<uint32_t> monomerOf;     // slot -> monomer (UINT32_MAX for padding)
     std::vector<uint32_t> pos;           // per slot, Z-curve code
+    // per slot, move_hash of the member's species-local index in the
+    // reference's order (CRITICAL.cu:322-323): the inner hash of its draw
+    std::vector<uint32_t> key;
     std::vector<int32_t> img[3];         // per slot, periodic images
     std::vector<uint32_t> nbr;           // neighbour matrices (slot ids)
     // Compressed partner ids: member j of species c has W 16-bit codes at

@@ -236,6 +236,11 @@ void Updater::setPerformWalk(uint32_t group, uint32_t depth) {
     if (initialized_) applyLaunchShape();
 }
 
+void Updater::setSpatialSlotOrder(bool spatial) {
+    if (initialized_) throw std::runtime_error("[setSpatialSlotOrder] call before initialize");
+    spatialOrder_ = spatial;
+}
+
 void Updater::setClearPass(uint32_t pass) {
     if (pass > 2) throw std::invalid_argument("[setClearPass] pass must be 0 (automatic), 1 or 2");
     clearPassOverride_ = pass;
@@ -346,6 +351,7 @@ SystemInput Updater::stagedInput() const {
         for (uint32_t k = 0; k < degree_[i]; ++k)
             in.bonds.ids[in.bonds.start[i] + k] = nbrIds_[size_t(i) * kMaxConnectivity + k];
     std::memcpy(in.forbidden, forbidden_, sizeof forbidden_);
+    in.spatialOrder = spatialOrder_;
     return in;
 }
 

@@ -61,9 +61,10 @@ def forbidden_table() -> np.ndarray:
 
 
 def stage_updater(s: Sys, seed: int, kind: int = P.LATTICE_BIT, colors=None, gpu: int = 0, bulk: bool = True,
-                  block: int = 0, slots: int = 0) -> P.UpdaterGPUScBFM_AB_Type:
+                  block: int = 0, slots: int = 0, spatial: bool = True) -> P.UpdaterGPUScBFM_AB_Type:
     """Stages a system in the call order of GPUScBFM_AB_Type.h:83-130, up to
-    (not including) initialize."""
+    (not including) initialize.  spatial: slot order inside each species
+    (Z-curve, the default, or monomer order)."""
     u = P.UpdaterGPUScBFM_AB_Type()
     u.setGpu(gpu)
     u.setPeriodicity(True, True, True)
@@ -89,13 +90,14 @@ def stage_updater(s: Sys, seed: int, kind: int = P.LATTICE_BIT, colors=None, gpu
         u.setColors(colors)
     if block or slots:
         u.setLaunchShape(block, slots)
+    u.setSlotOrder(spatial)
     return u
 
 
 def make_updater(s: Sys, seed: int, kind: int = P.LATTICE_BIT, colors=None, gpu: int = 0, bulk: bool = True,
-                 block: int = 0, slots: int = 0) -> P.UpdaterGPUScBFM_AB_Type:
+                 block: int = 0, slots: int = 0, spatial: bool = True) -> P.UpdaterGPUScBFM_AB_Type:
     """Configures and initializes an updater (GPUScBFM_AB_Type.h:83-130)."""
-    u = stage_updater(s, seed, kind, colors, gpu, bulk, block, slots)
+    u = stage_updater(s, seed, kind, colors, gpu, bulk, block, slots, spatial)
     u.initialize()
     return u
 
@@ -106,10 +108,10 @@ def oracle_for(s: Sys, colors, seed: int) -> oracle.System:
 
 
 def gpu_vs_oracle(s: Sys, seed: int, n_mcs: int, kind: int = P.LATTICE_BIT, chunks=(None,), use_graph=True,
-                  block: int = 0, slots: int = 0):
+                  block: int = 0, slots: int = 0, spatial: bool = True):
     """Runs n_mcs sweeps on the GPU (split into `chunks` calls) and on the
     oracle; returns (gpu_xyz, gpu_log, oracle_xyz, oracle_log)."""
-    u = make_updater(s, seed, kind, block=block, slots=slots)
+    u = make_updater(s, seed, kind, block=block, slots=slots, spatial=spatial)
     colors = u.colors(s.n)
     logs = []
     plan = [n_mcs] if chunks == (None,) else list(chunks)

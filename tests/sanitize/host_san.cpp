@@ -68,8 +68,27 @@ void forbidden_table(uint8_t forbidden[512]) {
         forbidden[(uint32_t(v[0]) & 7u) | ((uint32_t(v[1]) & 7u) << 3) | ((uint32_t(v[2]) & 7u) << 6)] = 0;
 }
 
-// Layout of the system (coloring included) with both coloring modes; the
-// per-slot codes and image counters must decode back to the input.
+// Slot order inside each species: spatial layouts have strictly increasing
+// codes, reference layouts increasing monomer indices; either way every
+// slot's key is move_hash of the member's rank among its species in
+// monomer-index order (the draw key of CRITICAL.cu:322-323).
+void check_slot_order(const SystemLayout& L, bool spatial, const std::string& name) {
+    std::vector<uint32_t> rank(L.nMonomers), next(L.colors.size(), 0);
+    for (uint32_t i = 0; i < L.nMonomers; ++i) rank[i] = next[L.color[i]]++;
+    bool keys = true, ordered = true;
+    for (uint32_t i = 0; i < L.nMonomers; ++i) keys = keys && L.key[L.slotOf[i]] == move_hash(rank[i]);
+    for (const ColorInfo& ci : L.colors)
+        for (uint32_t p = 1; p < ci.count; ++p) {
+            const uint32_t s = ci.slotOffset + p;
+            ordered = ordered && (spatial ? L.pos[s - 1] < L.pos[s] : L.monomerOf[s - 1] < L.monomerOf[s]);
+        }
+    expect(keys, name + ": slot keys are the hashes of the reference ranks");
+    expect(ordered, name + (spatial ? ": slots in Z-curve order" : ": slots in monomer order"));
+}
+
+// Layout of the system (coloring included) with both coloring modes and
+// both slot orders; the per-slot codes and image counters must decode back
+// to the input.
 std::vector<uint8_t> check_layout(const Sys& s, const char* name) {
     SystemInput in;
     std::memcpy(in.box, s.box, sizeof in.box);
@@ -77,6 +96,15 @@ std::vector<uint8_t> check_layout(const Sys& s, const char* name) {
     in.bonds = adjacency(s);
     forbidden_table(in.forbidden);
     std::vector<uint8_t> colors;
+    for (bool spatial : {false, true}) {
+        in.spatialOrder = spatial;
+        const SystemLayout L = build_layout(in, nullptr, true);
+        check_slot_order(L, spatial, name);
+        std::vector<int32_t> back(s.xyz.size());
+        const int32_t* img[3] = {L.img[0].data(), L.img[1].data(), L.img[2].data()};
+        L.unfolded(L.pos.data(), img, back.data());
+        expect(back == s.xyz, std::string(name) + ": layout in either slot order decodes to its input");
+    }
     for (bool uniform : {true, false}) {
         const SystemLayout L = build_layout(in, nullptr, uniform);
         std::vector<int32_t> back(s.xyz.size());

@@ -141,6 +141,17 @@ def test_large_replica_spread_over_all_xcds():
     assert b.info(small).live
 
 
+def test_batch_mixes_slot_orders():
+    # Replicas staged in Z-curve slot order and in monomer order share the
+    # launches; each ends where it ends alone.
+    systems = _mix()
+    b = P.Batch(tile_capacity=256, replica_capacity=16)
+    ids = [b.add(stage_updater(s, seed, spatial=(k % 2 == 0))) for k, (s, seed) in enumerate(systems)]
+    b.run(10)
+    for rid, (s, seed) in zip(ids, systems):
+        assert np.array_equal(b.positions(rid, s.n), _alone(s, seed, [10])[0])
+
+
 def test_read_back_in_rounds_equals_alone():
     # The batch decodes coordinates on the device into a buffer of at most
     # 64 MiB of records (5,592,405 monomers) and copies them back in rounds:

@@ -76,17 +76,18 @@ def test_largest_box_across_the_wrapping_corner():
 
 
 @pytest.mark.parametrize("kind", KINDS)
-def test_crosslinked_network_with_scattered_partners(kind):
+@pytest.mark.parametrize("spatial", [True, False])
+def test_crosslinked_network_with_scattered_partners(kind, spatial):
     # 1000 four-arm stars grown into a connected network: a large share of the
     # bonds join monomers whose slots lie far from the row's common partner
     # base, so the uint8 kernel's 16-bit partner codes fall back to the full
-    # neighbour matrix for them
+    # neighbour matrix for them (in either slot order; most in monomer order)
     s = stars((128, 128, 128), 1000, 4, 29, seed=7)
-    print(f"network kind {kind}: {s.n} monomers", flush=True)
+    print(f"network kind {kind} spatial {spatial}: {s.n} monomers", flush=True)
     from helpers import stage_updater
-    partners, escapes = stage_updater(s, 17, kind).partnerEscapes()
+    partners, escapes = stage_updater(s, 17, kind, spatial=spatial).partnerEscapes()
     assert 0 < escapes < partners  # the escape path is taken
-    gx, glog, ox, olog = gpu_vs_oracle(s, seed=17, n_mcs=30, kind=kind)
+    gx, glog, ox, olog = gpu_vs_oracle(s, seed=17, n_mcs=30, kind=kind, spatial=spatial)
     assert np.array_equal(glog, olog)
     assert np.array_equal(gx, ox)
     assert glog.sum() > 0

@@ -128,6 +128,29 @@ def test_clear_pass_does_not_change_results(kind, walk):
     assert np.array_equal(x, o.xyz)
 
 
+@pytest.mark.parametrize("kind", [P.LATTICE_BIT, P.LATTICE_U8])
+@pytest.mark.parametrize("system", ["network", "melt"])
+def test_slot_order_does_not_change_results(kind, system):
+    # Slots of a species in Z-curve order of the starting corners (the
+    # default) or in monomer order: every move is drawn from the member's
+    # index in the reference's order either way (CRITICAL.cu:322-323), so
+    # both runs equal the oracle's, with the same species.
+    s = stars((64, 64, 64), 40, 4, 29, seed=9) if system == "network" else ordered_chains((64, 64, 64), 64, 32)
+    runs = []
+    for spatial in (True, False):
+        u = make_updater(s, 21, kind, spatial=spatial)
+        _, log = u.run(12, log=True)
+        runs.append((u.getPositions(s.n), log, u.colors(s.n)))
+        u.cleanup()
+    colors = runs[0][2]
+    o = oracle_for(s, colors, 21)
+    _, olog = o.run(0, 12 * (int(colors.max()) + 1))
+    for x, log, c in runs:
+        assert np.array_equal(c, colors)
+        assert np.array_equal(log, olog)
+        assert np.array_equal(x, o.xyz)
+
+
 def test_single_monomer_always_moves():
     s = Sys((16, 16, 16), np.array([[3, 4, 5]], np.int32), np.zeros((0, 2), np.uint32))
     u = make_updater(s, seed=0)

@@ -498,11 +498,16 @@ def test_partner_codes_of_chains_need_no_escape():
 
 def test_crosslinked_network_has_partner_escapes():
     # the network of the GPU test of scattered partners: crosslinks join
-    # monomers whose slots lie far apart, which the 16-bit codes cannot span
+    # monomers whose slots lie far apart, or in another species than the
+    # row's common base, which the 16-bit codes cannot span; in either slot
+    # order
     s = stars((128, 128, 128), 1000, 4, 29, seed=7)
-    partners, escapes = _staged_without_gpu(s).partnerEscapes()
-    assert partners == 2 * s.bonds.shape[0]
-    assert 0 < escapes < partners
+    for spatial in (False, True):
+        u = _staged_without_gpu(s)
+        u.setSlotOrder(spatial)
+        partners, escapes = u.partnerEscapes()
+        assert partners == 2 * s.bonds.shape[0]
+        assert 0 < escapes < partners
 
 
 def test_perform_walk_is_validated():
