@@ -396,7 +396,7 @@ def main() -> int:
     ap.add_argument("--serve-rates", default="20,80,160,320",
                     help="Poisson arrival rates (jobs/s) of the served stream")
     ap.add_argument("--serve-seconds", type=float, default=8.0, help="arrival window per rate")
-    ap.add_argument("--pressure-budget-gb", type=float, default=0.6,
+    ap.add_argument("--pressure-budget-gb", type=float, default=0.7,
                     help="HBM budget of the stream where admission by footprint binds (0: skip it)")
     ap.add_argument("--pressure-melts", type=int, default=6, help="512^3 melts in the budget-bound stream")
     ap.add_argument("--pressure-melt-mcs", type=int, default=20000, help="sweeps each of those melts asks for")

@@ -353,7 +353,7 @@ int pscbfm_batch_usage(pscbfm_batch* b, uint32_t* live, uint32_t* used_tiles, ui
 
 /* ---- Request loop (continuous batching) -------------------------------------
  * No counterpart in the reference.  Jobs (a staged updater + a number of
- * sweeps) are submitted from any thread; a preparation thread builds their
+ * sweeps) are submitted from any thread; four preparation threads build their
  * layouts, and a GPU thread admits them into one batch as HBM budget,
  * workgroups and replica slots allow, launches the live replicas at most
  * `quantum` substeps at a time (so arrivals join while others run), and

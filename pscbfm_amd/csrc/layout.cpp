@@ -127,6 +127,26 @@ void encode_partner_ids(const SystemInput& in, SystemLayout& L) {
     }
 }
 
+// Stable LSD radix sort of packed (code << 32 | rank) words by code, whose
+// significant bits are the low `bits`: words of equal code keep their order.
+void sort_by_code(std::vector<uint64_t>& a, uint32_t bits) {
+    constexpr uint32_t kDigit = 11, kRadix = 1u << kDigit;
+    std::vector<uint64_t> tmp(a.size());
+    std::vector<size_t> start(kRadix);
+    for (uint32_t shift = 0; shift < bits; shift += kDigit) {
+        std::fill(start.begin(), start.end(), size_t(0));
+        for (uint64_t v : a) ++start[uint32_t(v >> (32 + shift)) & (kRadix - 1)];
+        size_t sum = 0;
+        for (size_t& c : start) {
+            const size_t k = c;
+            c = sum;
+            sum += k;
+        }
+        for (uint64_t v : a) tmp[start[uint32_t(v >> (32 + shift)) & (kRadix - 1)]++] = v;
+        a.swap(tmp);
+    }
+}
+
 }  // namespace
 
 SystemLayout build_layout(const SystemInput& in, const std::vector<uint8_t>* colorOverride,
@@ -212,7 +232,7 @@ SystemLayout build_layout(const SystemInput& in, const std::vector<uint8_t>* col
         if (in.spatialOrder) {
             std::vector<uint64_t> packed(order.size());
             for (uint32_t r = 0; r < order.size(); ++r) packed[r] = (uint64_t(code[members[c][r]]) << 32) | r;
-            std::sort(packed.begin(), packed.end());
+            sort_by_code(packed, L.codec.log2L[0] + L.codec.log2L[1] + L.codec.log2L[2]);
             for (uint32_t r = 0; r < order.size(); ++r) order[r] = uint32_t(packed[r]);
         }
     };

@@ -14,8 +14,21 @@ Server::Server(int gpu, uint32_t latticeKind, uint64_t hbmBudget, uint32_t tileC
     batch_.reset(new Batch(gpu, latticeKind, tileCapacity, replicaCapacity, block, slots));
     stats_.tileCapacity = batch_->tileCapacity();
     t0_ = std::chrono::steady_clock::now();
-    prep_ = std::thread([this] { prepLoop(); });
-    worker_ = std::thread([this] { gpuLoop(); });
+    try {
+        for (unsigned k = 0; k < kPrepThreads; ++k) prep_.emplace_back([this] { prepLoop(); });
+        worker_ = std::thread([this] { gpuLoop(); });
+    } catch (...) {
+        {
+            std::lock_guard<std::mutex> lock(mu_);
+            stop_ = true;
+        }
+        cvPrep_.notify_all();
+        cvGpu_.notify_all();
+        for (std::thread& t : prep_)
+            if (t.joinable()) t.join();
+        if (worker_.joinable()) worker_.join();
+        throw;
+    }
 }
 
 Server::~Server() {
@@ -25,7 +38,8 @@ Server::~Server() {
     }
     cvPrep_.notify_all();
     cvGpu_.notify_all();
-    if (prep_.joinable()) prep_.join();
+    for (std::thread& t : prep_)
+        if (t.joinable()) t.join();
     if (worker_.joinable()) worker_.join();
 }
 
@@ -94,7 +108,11 @@ void Server::prepLoop() {
             continue;
         }
         r.state = JobState::Prepared;
-        prepared_.push_back(std::move(p));
+        // in arrival order (ids grow with submission), whichever thread
+        // finished first
+        auto at = std::upper_bound(prepared_.begin(), prepared_.end(), p->id,
+                                   [](uint64_t id, const std::unique_ptr<Pending>& q) { return id < q->id; });
+        prepared_.insert(at, std::move(p));
         cvGpu_.notify_one();
     }
 }

@@ -1,7 +1,8 @@
 // Continuous-batching request loop over one Batch.
 //
 // Jobs (a staged system + a number of sweeps) are submitted from any thread.
-// A preparation thread builds their layouts; the GPU thread admits prepared
+// A few preparation threads build their layouts (a large system does not hold
+// up the small ones behind it); the GPU thread admits prepared
 // jobs whenever their device footprint fits the HBM budget and their
 // workgroups fit the tile table, launches the live replicas up to the next
 // completion (at most `quantum` substeps at a time, so that arrivals are
@@ -99,7 +100,9 @@ private:
     ServerStats stats_;
     std::string fatal_;
 
-    std::thread prep_, worker_;
+    static constexpr unsigned kPrepThreads = 4;
+    std::vector<std::thread> prep_;
+    std::thread worker_;
 };
 
 }  // namespace pscbfm
