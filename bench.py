@@ -640,7 +640,7 @@ def main() -> int:
             jobs = serving.draw_jobs(rate, args.serve_seconds, args.seed + 100 + k)
             log(f"served stream at {rate:g} jobs/s: {len(jobs)} jobs "
                 f"({sum(j.kind == 'melt' for j in jobs)} dense 512^3 melts)")
-            res = serving.run_stream(jobs, log=log, seed=args.seed)
+            res = serving.run_stream(jobs, log=log, seed=args.seed, spatial_order=args.slot_order == "spatial")
             log(f"  {res['jobs_per_s']:.1f} jobs/s, latency p50 {res['latency_p50_s']:.2f} s "
                 f"p99 {res['latency_p99_s']:.2f} s, queue wait p99 {res['queue_wait_p99_s']:.2f} s, GPU busy "
                 f"{res['gpu_busy_fraction']:.2f}, peak resident {res['peak_resident_GB']:.2f} GB; {res['parity']} "
@@ -662,7 +662,8 @@ def main() -> int:
             budget = int(args.pressure_budget_gb * 1e9)
             log(f"served stream at {rate:g} jobs/s under a {args.pressure_budget_gb:g} GB budget: {len(jobs)} jobs "
                 f"({sum(j.kind == 'melt' for j in jobs)} dense 512^3 melts)")
-            res = serving.run_stream(jobs, hbm_budget=budget, log=log, seed=args.seed)
+            res = serving.run_stream(jobs, hbm_budget=budget, log=log, seed=args.seed,
+                                     spatial_order=args.slot_order == "spatial")
             log(f"  {res['jobs_per_s']:.1f} jobs/s, latency p50 {res['latency_p50_s']:.2f} s "
                 f"p99 {res['latency_p99_s']:.2f} s, queue wait p99 {res['queue_wait_p99_s']:.2f} s, "
                 f"peak resident {res['peak_resident_GB']:.3f} of {args.pressure_budget_gb:g} GB; {res['parity']}; "
