@@ -460,19 +460,17 @@ void Updater::setGraphChunk(uint32_t substeps) {
     graphChunk_ = substeps;
 }
 
+// The bulk graph as often as it fits, then one graph of exactly the
+// remaining substeps: every graph launch costs tens of microseconds of device
+// time before its first kernel, so a short run is one launch.  Graphs are
+// cached per size; a caller that repeats one run length builds at most two.
 Updater::GraphPlan Updater::planGraphs(uint64_t S) {
     GraphPlan p;
     const uint32_t G = graphChunk();
     p.bulkCount = S / G;
     if (p.bulkCount) p.bulk = graphFor(G);
-    uint64_t left = S - p.bulkCount * G;
-    uint64_t g = 1;
-    while (g * 2u <= G) g *= 2u;
-    for (; g >= 1u && left > 0; g >>= 1)
-        if (left >= g) {
-            p.tail.push_back(graphFor(uint32_t(g)));
-            left -= g;
-        }
+    const uint64_t left = S - p.bulkCount * G;
+    if (left) p.tail.push_back(graphFor(uint32_t(left)));
     return p;
 }
 
@@ -642,8 +640,8 @@ void Updater::runChunk(uint64_t S, uint32_t* log, bool profile, bool useGraph, v
     } else {
         uint64_t left = S;
         if (useGraph) {
-            // the bulk graph, then the binary digits of the remainder (each
-            // below the bulk size), so every substep runs from a graph
+            // the bulk graph, then the remainder's graph, so every substep
+            // runs from a graph
             for (uint64_t r = 0; r < plan.bulkCount; ++r)
                 hip_check(hipGraphLaunch(plan.bulk, stream_), "hipGraphLaunch");
             for (hipGraphExec_t g : plan.tail) hip_check(hipGraphLaunch(g, stream_), "hipGraphLaunch");

@@ -84,7 +84,7 @@ public:
     // Threads per workgroup (0 = 256) and monomers per pass-1 thread (0 = 2).
     void setLaunchShape(uint32_t block, uint32_t slots);
     // Substeps of the bulk graph (0 = nColors * max(1, 256 / nColors)); a
-    // run replays it, then power-of-two graphs cover the remainder.
+    // run replays it, then one graph of the remaining substeps.
     // invalid_argument above kMaxGraphChunk (each substep is two kernel nodes).
     static constexpr uint32_t kMaxGraphChunk = 4096;
     void setGraphChunk(uint32_t substeps);
@@ -182,7 +182,7 @@ private:
     // instantiated and uploaded on first use.
     hipGraphExec_t graphFor(uint32_t substeps);
     // The replays that cover S substeps: `bulkCount` times the bulk graph,
-    // then each graph of `tail` once (the binary digits of the remainder).
+    // then each graph of `tail` once (the remainder's, if any).
     struct GraphPlan {
         hipGraphExec_t bulk = nullptr;
         uint64_t bulkCount = 0;
