@@ -71,8 +71,14 @@ void parallel_for(uint32_t n, F&& f) {
     }
     std::vector<std::thread> pool;
     const uint32_t chunk = (n + nThreads - 1) / nThreads;
-    for (uint32_t t = 1; t < nThreads; ++t)
-        pool.emplace_back([&f, t, chunk, n] { f(std::min(n, t * chunk), std::min(n, (t + 1) * chunk)); });
+    for (uint32_t t = 1; t < nThreads; ++t) {
+        auto part = [&f, t, chunk, n] { f(std::min(n, t * chunk), std::min(n, (t + 1) * chunk)); };
+        try {
+            pool.emplace_back(part);
+        } catch (...) {
+            part();  // no thread available: this part on the calling thread
+        }
+    }
     f(0u, std::min(n, chunk));
     for (std::thread& th : pool) th.join();
 }
