@@ -196,6 +196,22 @@ int main(int argc, char** argv) {
             for (size_t k = 0; k + 1 < pairs.size(); k += 2) s.bonds.push_back({pairs[k], pairs[k + 1]});
             check_oracle(s, check_layout(s, "star network"), "star network");
         }
+        // non-cubic boxes: the Z-curve then interleaves fewer bits on the
+        // short axes, and the slot sort covers codes of fewer bits
+        {
+            Sys s{{16, 32, 64}, {}, {}};
+            expect(grow_linear_chains(s.box, 12, 24, 11, 200, s.xyz), "grown chains, 16x32x64");
+            for (uint32_t c = 0; c < 12; ++c)
+                for (uint32_t i = 0; i + 1 < 24; ++i) s.bonds.push_back({c * 24 + i, c * 24 + i + 1});
+            check_oracle(s, check_layout(s, "grown chains, 16x32x64"), "grown chains, 16x32x64");
+        }
+        {
+            Sys s{{128, 32, 64}, {}, {}};
+            std::vector<uint32_t> pairs;
+            expect(build_star_network(s.box, 30, 4, 19, 5, s.xyz, pairs), "star network, 128x32x64");
+            for (size_t k = 0; k + 1 < pairs.size(); k += 2) s.bonds.push_back({pairs[k], pairs[k + 1]});
+            check_oracle(s, check_layout(s, "star network, 128x32x64"), "star network, 128x32x64");
+        }
         // the reference's fixtures: read, lay out, write and read back
         for (const char* f : {"LinearChain_l1024_n64.bfm.gz", "Ring_N6_n6.bfm", "Hydrogel_NStar_117_NoPerXYZ64.bfm"}) {
             const BfmData b = read_bfm(golden + "/" + f);
