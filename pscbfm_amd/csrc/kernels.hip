@@ -11,17 +11,21 @@
 //    Z-order bond key of two codes;
 //  * both passes are bound by the number of gather instructions and memory
 //    latency more than by bandwidth: a pass-1 thread handles SLOTS monomers
-//    and issues all their loads in two batches (own codes and partner ids;
-//    then partner codes and the front plane together, from one basic block);
-//    the 3x3 front plane is read as at most four 64-bit words;
+//    and issues all their loads in two batches (own codes, draw keys and
+//    partner ids; then partner codes and the front plane together, from one
+//    basic block); the 3x3 front plane is read as at most four 64-bit words;
+//  * a species' slots may be in the Z-curve order of the starting corners,
+//    so that a wave's members and their partners and lattice words lie
+//    close together; each slot's key is the inner move hash of the member's
+//    rank in the reference's order, which keys its draw either way;
 //  * pass 1 compacts the surviving proposals of its workgroup into a list
 //    (wave ballot + LDS counter; the order inside a list is irrelevant
 //    because the result of a substep does not depend on processing order),
 //    and pass 2 walks those lists densely;
 //  * the proposal lattice is 1 bit per cell and double-buffered by substep
 //    parity: pass 1 of substep t sets bits in buffer t&1, pass 2 of substep t
-//    reads it and zeroes buffer (t+1)&1 (last read by pass 2 of substep t-1,
-//    next written by pass 1 of substep t+1);
+//    reads it, and pass 1 or pass 2 of substep t zeroes buffer (t+1)&1 (last
+//    read by pass 2 of substep t-1, next written by pass 1 of substep t+1);
 //  * the occupancy lattice is updated with plain byte stores (no atomics):
 //    device-scope atomics are executed beyond the per-XCD L2 and cost a
 //    memory transaction each;
