@@ -17,8 +17,8 @@ pytestmark = pytest.mark.gpu
 EXE = Path(P.__file__).parent / "pscbfm_run"
 
 
-def _resumed(s, seed, start, mcs, colors=None, kind=P.LATTICE_BIT):
-    u = stage_updater(s, seed, kind, colors=colors)
+def _resumed(s, seed, start, mcs, colors=None, kind=P.LATTICE_BIT, spatial=True):
+    u = stage_updater(s, seed, kind, colors=colors, spatial=spatial)
     u.setStartMcs(start)
     u.initialize()
     st, log = u.run(mcs, log=True)
@@ -46,6 +46,22 @@ def test_resumed_updater_continues_the_trajectory(kind):
     o = oracle_for(Sys(s.box, mid, s.bonds), colors, 31)
     _, olog = o.run(12 * nc, 9 * nc)
     assert np.array_equal(olog, log_b) and np.array_equal(o.xyz, end)
+
+
+@pytest.mark.parametrize("first_spatial", [True, False])
+def test_resume_across_slot_orders(first_spatial):
+    # The slot order is invisible in results: a run in one order, resumed
+    # from its coordinates in the other, continues the uninterrupted run.
+    s = stars((64, 64, 64), 20, 4, 10, seed=5)
+    u = make_updater(s, 13, spatial=first_spatial)
+    colors = u.colors(s.n)
+    u.run(7)
+    mid = u.getPositions(s.n)
+    _, log_b = u.run(8, log=True)
+    end = u.getPositions(s.n)
+    u.cleanup()
+    x, log, _ = _resumed(Sys(s.box, mid, s.bonds), 13, 7, 8, colors, spatial=not first_spatial)
+    assert np.array_equal(log, log_b) and np.array_equal(x, end)
 
 
 def test_bfm_checkpoint_round_trip(tmp_path):
