@@ -50,7 +50,10 @@ def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def make_updater(box, xyz, bonds, seed, kind, block=0, slots=0, spatial_order=True):
+SLOT_ORDER = {"auto": None, "spatial": True, "reference": False}
+
+
+def make_updater(box, xyz, bonds, seed, kind, block=0, slots=0, spatial_order=None):
     sd = W.SystemData(tuple(box), xyz, bonds, seed, "ordered")
     return W.stage(sd, kind, block=block, slots=slots, initialize=True, spatial_order=spatial_order)
 
@@ -160,7 +163,7 @@ def cpu_single_core(u, box, bonds, colors, seed, target_seconds: float) -> dict:
 
 
 def network_section(seed: int, mcs: int, kind: int, c_l: float, shape=None, clear_pass: int = 0,
-                    spatial_order: bool = True) -> dict:
+                    spatial_order=None) -> dict:
     """Star network: 12800 4-arm stars of arm length 29 (1,497,600 monomers)
     with arm ends crosslinked to other stars, on a 512x256x256 periodic box
     (occupied fraction 0.357; the same monomer count on 256^3 would be 0.72,
@@ -364,7 +367,7 @@ def main() -> int:
     ap.add_argument("--list-group", type=int, default=0,
                     help="pass-2 threads per pass-1 list (0: default; -1: one list per workgroup)")
     ap.add_argument("--list-depth", type=int, default=0, help="pass-2 entries per thread in flight (0: default)")
-    ap.add_argument("--slot-order", choices=["spatial", "reference"], default="spatial",
+    ap.add_argument("--slot-order", choices=["auto", "spatial", "reference"], default="auto",
                     help="slot order inside each species (results are the same)")
     ap.add_argument("--clear-pass", type=int, default=0, choices=[0, 1, 2],
                     help="pass that clears the other parity's proposals (0: default)")
@@ -422,7 +425,7 @@ def main() -> int:
     xyz = P.gen_linear_chains_ordered(box, n_chains, args.chain)
     bonds = P.chain_bonds(n_chains, args.chain)
     u = make_updater(box, xyz, bonds, args.seed, kind, args.block, args.slots,
-                     spatial_order=args.slot_order == "spatial")
+                     spatial_order=SLOT_ORDER[args.slot_order])
     if args.perform_block:
         u.setPerformBlock(args.perform_block)
     if args.graph_chunk:
@@ -578,7 +581,7 @@ def main() -> int:
         okind = P.LATTICE_U8 if other == "u8" else P.LATTICE_BIT
         oc_l = 1.0 if other == "u8" else 0.125
         v = make_updater(box, xyz, bonds, args.seed, okind, args.block, args.slots,
-                         spatial_order=args.slot_order == "spatial")
+                         spatial_order=SLOT_ORDER[args.slot_order])
         same_mcs = args.prerelax + args.warmup + args.steps
         v.run(same_mcs)
         cross = bool(np.array_equal(v.getPositions(n), timed_xyz))
@@ -625,7 +628,7 @@ def main() -> int:
     if not args.no_network and args.network_mcs > 0:
         nshape = tuple(int(v) for v in args.network_shape.split("x")) if args.network_shape else None
         out["network"] = network_section(args.seed, args.network_mcs, kind, c_l, nshape, args.clear_pass,
-                                         args.slot_order == "spatial")
+                                         SLOT_ORDER[args.slot_order])
     if not args.no_batch and args.replicas > 0:
         out["batched"] = batched_vs_plain(args.replicas, args.replica_mcs, args.seed, kind, args.tune_sample,
                                           args.tune_mcs, plain=not args.no_plain,
@@ -640,7 +643,7 @@ def main() -> int:
             jobs = serving.draw_jobs(rate, args.serve_seconds, args.seed + 100 + k)
             log(f"served stream at {rate:g} jobs/s: {len(jobs)} jobs "
                 f"({sum(j.kind == 'melt' for j in jobs)} dense 512^3 melts)")
-            res = serving.run_stream(jobs, log=log, seed=args.seed, spatial_order=args.slot_order == "spatial")
+            res = serving.run_stream(jobs, log=log, seed=args.seed, spatial_order=SLOT_ORDER[args.slot_order])
             log(f"  {res['jobs_per_s']:.1f} jobs/s, latency p50 {res['latency_p50_s']:.2f} s "
                 f"p99 {res['latency_p99_s']:.2f} s, queue wait p99 {res['queue_wait_p99_s']:.2f} s, GPU busy "
                 f"{res['gpu_busy_fraction']:.2f}, peak resident {res['peak_resident_GB']:.2f} GB; {res['parity']} "
@@ -663,7 +666,7 @@ def main() -> int:
             log(f"served stream at {rate:g} jobs/s under a {args.pressure_budget_gb:g} GB budget: {len(jobs)} jobs "
                 f"({sum(j.kind == 'melt' for j in jobs)} dense 512^3 melts)")
             res = serving.run_stream(jobs, hbm_budget=budget, log=log, seed=args.seed,
-                                     spatial_order=args.slot_order == "spatial")
+                                     spatial_order=SLOT_ORDER[args.slot_order])
             log(f"  {res['jobs_per_s']:.1f} jobs/s, latency p50 {res['latency_p50_s']:.2f} s "
                 f"p99 {res['latency_p99_s']:.2f} s, queue wait p99 {res['queue_wait_p99_s']:.2f} s, "
                 f"peak resident {res['peak_resident_GB']:.3f} of {args.pressure_budget_gb:g} GB; {res['parity']}; "

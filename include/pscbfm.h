@@ -151,13 +151,16 @@ int pscbfm_set_perform_block(pscbfm_updater* h, uint32_t block);
  * threads_per_list = 0 or 0xFFFFFFFF (the default) walks one list per
  * workgroup.  Results do not depend on it. */
 int pscbfm_set_perform_walk(pscbfm_updater* h, uint32_t threads_per_list, uint32_t entries_per_thread);
-/* Slot order inside each species, before initialize: spatial = 1 (the
- * default) places the members of a species in the Z-curve order of their
- * starting corners, spatial = 0 in monomer-index order as the reference
- * (UpdaterGPUScBFM_AB_Type.cu:601-607, iToiNew).  Each move is drawn from the
- * member's index in the reference's order either way (CRITICAL.cu:322-323),
- * so results do not depend on it. */
-int pscbfm_set_slot_order(pscbfm_updater* h, int spatial);
+/* Slot order inside each species, before initialize: 1 places the members
+ * of a species in the Z-curve order of their starting corners, 0 in
+ * monomer-index order as the reference (UpdaterGPUScBFM_AB_Type.cu:601-607,
+ * iToiNew), 2 (the default) chooses: 1 on the bit lattice, and on the byte
+ * lattice 1 only when the monomer order scatters bonded partners (species-
+ * local indices more than 32766 apart for at least 1% of the bonds).  Each
+ * move is drawn from the member's index in the reference's order either way
+ * (CRITICAL.cu:322-323), so results do not depend on it.  Other values:
+ * PSCBFM_INVALID_ARGUMENT. */
+int pscbfm_set_slot_order(pscbfm_updater* h, int order);
 /* The pass of each substep (1 or 2; 0 = automatic: pass 1 on the bit
  * lattice with at most two bonds per monomer, else pass 2) that zeroes the
  * other parity's proposal buffer, the reference's separate clear

@@ -377,10 +377,16 @@ class UpdaterGPUScBFM_AB_Type:
         one list per workgroup) and entries in flight per thread (1 or 4; 0: 4)."""
         self._call("pscbfm_set_perform_walk", int(threads_per_list), int(entries_per_thread))
 
-    def setSlotOrder(self, spatial: bool = True) -> None:
-        """Slot order inside each species (before initialize): spatial, by
-        starting Z-curve code (default), or the reference's monomer order."""
-        self._call("pscbfm_set_slot_order", int(bool(spatial)))
+    def setSlotOrder(self, order=None) -> None:
+        """Slot order inside each species (before initialize): "spatial" (or
+        True), by starting Z-curve code; "reference" (or False), the
+        reference's monomer order; None or "auto" (the default), spatial on
+        the bit lattice and on the byte lattice only for scattered partners.
+        Results are the same in every order."""
+        codes = {None: 2, "auto": 2, True: 1, "spatial": 1, False: 0, "reference": 0}
+        if order not in codes:
+            raise InvalidArgument(f"unknown slot order {order!r}")
+        self._call("pscbfm_set_slot_order", codes[order])
 
     def setClearPass(self, pass_: int = 0) -> None:
         """The pass (1 or 2; 0: automatic, pass 1 on the bit lattice with at

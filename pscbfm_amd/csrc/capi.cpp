@@ -297,8 +297,8 @@ int pscbfm_set_perform_walk(pscbfm_updater* h, uint32_t threads_per_list, uint32
     return guarded(h, [&](pscbfm::Updater& u) { u.setPerformWalk(threads_per_list, entries_per_thread); });
 }
 
-int pscbfm_set_slot_order(pscbfm_updater* h, int spatial) {
-    return guarded(h, [&](pscbfm::Updater& u) { u.setSpatialSlotOrder(spatial != 0); });
+int pscbfm_set_slot_order(pscbfm_updater* h, int order) {
+    return guarded(h, [&](pscbfm::Updater& u) { u.setSlotOrder(static_cast<pscbfm::SlotOrder>(order)); });
 }
 
 int pscbfm_set_clear_pass(pscbfm_updater* h, uint32_t pass) {

@@ -278,6 +278,7 @@ struct SysDesc {
     uint32_t maxDegree;      // largest bond count of any monomer
     uint32_t accEntries;     // entries of state->acceptedByBlock
     uint32_t nSlots;         // slots of pos (decoded ids are clamped below it)
+    uint32_t keyed;          // 1: slots in spatial order, read the draw keys; 0: hash the slot index
 };
 
 }  // namespace pscbfm

@@ -117,6 +117,7 @@ void DeviceSystem::allocate(const SystemLayout& L, uint32_t latticeKind, uint64_
     desc.ids = reinterpret_cast<const uint32_t*>(base + p.off[P_IDS]);
     desc.rowBase = reinterpret_cast<const uint32_t*>(base + p.off[P_ROWB]);
     desc.key = reinterpret_cast<const uint32_t*>(base + p.off[P_KEY]);
+    desc.keyed = L.spatialOrder ? 1u : 0u;
     desc.nSlots = std::max<uint32_t>(L.nSlots, 1u);
     desc.state = reinterpret_cast<DevState*>(base + p.off[P_STATE]);
     desc.seed = seed;

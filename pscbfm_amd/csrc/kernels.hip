@@ -247,8 +247,10 @@ __device__ __forceinline__ void check_body(const SysDesc& d, uint64_t t, Substep
     const gptr<const uint32_t> pos = posAll + ci.slotOffset;
     const gptr<const uint32_t> nbr = d.nbr.global();
     // Inner hash of each member's species-local index in the reference's
-    // order, the key of its draw (slots may be in spatial order).
+    // order, the key of its draw: read from the slot's key when the slots
+    // are in spatial order, else the hash of the slot index itself.
     const gptr<const uint32_t> keys = d.key.global() + ci.slotOffset;
+    const bool keyed = d.keyed != 0u;
     // Partner ids: the bit-lattice kernel reads the full matrix rows; the
     // uint8-lattice kernel, whose plane words cost more traffic, reads the
     // 16-bit codes (half the bytes) and the full matrix only for escapes.
@@ -273,7 +275,10 @@ __device__ __forceinline__ void check_body(const SysDesc& d, uint64_t t, Substep
         live[s] = j[s] < ci.count;
         const uint32_t jj = live[s] ? j[s] : 0u;  // count > 0 for every species
         m[s] = pos[jj];
-        hj[s] = keys[jj];
+        if (keyed)
+            hj[s] = keys[jj];
+        else
+            hj[s] = move_hash(jj);
         if constexpr (LAT == kLatticeU8) {
 #pragma unroll
             for (int q = 0; q < MAXDEG / 2; ++q) {

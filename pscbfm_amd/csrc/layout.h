@@ -10,17 +10,24 @@
 
 namespace pscbfm {
 
+// Slot order inside each species (see SystemInput::slotOrder).
+enum class SlotOrder : int { Reference = 0, Spatial = 1, Automatic = 2 };
+
 struct SystemInput {
     uint32_t box[3] = {0, 0, 0};
     std::vector<int32_t> xyz;        // unfolded corner, 3 per monomer
     Adjacency bonds;                 // directed bond lists as registered
     uint8_t forbidden[512];          // reference bond table, index (dx&7)|(dy&7)<<3|(dz&7)<<6
-    // Slot order inside each species: false = by monomer index (the
-    // reference's order); true = by the Z-curve code of the starting corner,
-    // so that a wave's members, their partners and their lattice words lie
-    // close together.  Results do not depend on it: a move's draw is keyed by
-    // the member's rank in the reference's order either way.
-    bool spatialOrder = true;
+    // Slot order inside each species: Reference = by monomer index (the
+    // reference's order); Spatial = by the Z-curve code of the starting
+    // corner, so that a wave's members, their partners and their lattice
+    // words lie close together; Automatic = Spatial on the bit lattice, and
+    // on the byte lattice only when the reference's order scatters bonded
+    // partners (their species-local indices differ by more than 32766 for
+    // at least 1% of the bonds).  Results do not depend on it: a move's draw
+    // is keyed by the member's rank in the reference's order either way.
+    SlotOrder slotOrder = SlotOrder::Automatic;
+    uint32_t latticeKind = kLatticeBit;  // for SlotOrder::Automatic
 };
 
 struct SystemLayout {
@@ -33,8 +40,11 @@ struct SystemLayout {
     std::vector<uint32_t> monomerOf;     // slot -> monomer (UINT32_MAX for padding)
     std::vector<uint32_t> pos;           // per slot, Z-curve code
     // per slot, move_hash of the member's species-local index in the
-    // reference's order (CRITICAL.cu:322-323): the inner hash of its draw
+    // reference's order (CRITICAL.cu:322-323): the inner hash of its draw.
+    // In the reference's order that index is the slot's own index in its
+    // species, and the kernels hash it instead of reading the key.
     std::vector<uint32_t> key;
+    bool spatialOrder = false;
     std::vector<int32_t> img[3];         // per slot, periodic images
     std::vector<uint32_t> nbr;           // neighbour matrices (slot ids)
     // Compressed partner ids: member j of species c has W 16-bit codes at

@@ -236,9 +236,11 @@ void Updater::setPerformWalk(uint32_t group, uint32_t depth) {
     if (initialized_) applyLaunchShape();
 }
 
-void Updater::setSpatialSlotOrder(bool spatial) {
-    if (initialized_) throw std::runtime_error("[setSpatialSlotOrder] call before initialize");
-    spatialOrder_ = spatial;
+void Updater::setSlotOrder(SlotOrder order) {
+    if (initialized_) throw std::runtime_error("[setSlotOrder] call before initialize");
+    if (order != SlotOrder::Reference && order != SlotOrder::Spatial && order != SlotOrder::Automatic)
+        throw std::invalid_argument("[setSlotOrder] order must be 0 (reference), 1 (spatial) or 2 (automatic)");
+    slotOrder_ = order;
 }
 
 void Updater::setClearPass(uint32_t pass) {
@@ -351,7 +353,8 @@ SystemInput Updater::stagedInput() const {
         for (uint32_t k = 0; k < degree_[i]; ++k)
             in.bonds.ids[in.bonds.start[i] + k] = nbrIds_[size_t(i) * kMaxConnectivity + k];
     std::memcpy(in.forbidden, forbidden_, sizeof forbidden_);
-    in.spatialOrder = spatialOrder_;
+    in.slotOrder = slotOrder_;
+    in.latticeKind = latticeKind_;
     return in;
 }
 

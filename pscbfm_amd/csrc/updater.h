@@ -100,10 +100,10 @@ public:
     // kLegacyWalk (the default) walks one list per workgroup.
     static constexpr uint32_t kLegacyWalk = 0xFFFFFFFFu;
     void setPerformWalk(uint32_t group, uint32_t depth);
-    // Slot order inside each species (before initialize): spatial (default,
-    // by starting Z-curve code) or the reference's (by monomer index).
-    // Results do not depend on it.
-    void setSpatialSlotOrder(bool spatial);
+    // Slot order inside each species (before initialize): automatic (the
+    // default), spatial (by starting Z-curve code) or the reference's (by
+    // monomer index); see SystemInput::slotOrder.  Results do not depend on it.
+    void setSlotOrder(SlotOrder order);
     // The pass (1 or 2) that zeroes the other parity's proposal buffer each
     // substep; 0 = automatic (pass 1 on the bit lattice with at most two
     // bonds per monomer, else pass 2).
@@ -219,7 +219,7 @@ private:
     uint32_t blockOverride_ = 0, slotsOverride_ = 0, performOverride_ = 0;
     uint32_t walkGroupOverride_ = 0, walkDepthOverride_ = 0;
     uint32_t clearPassOverride_ = 0;
-    bool spatialOrder_ = true;
+    SlotOrder slotOrder_ = SlotOrder::Automatic;
 
     // device
     bool initialized_ = false;

@@ -69,7 +69,7 @@ def draw_jobs(rate: float, duration: float, seed: int, mcs_lo: int = 1000, mcs_h
 def run_stream(jobs: List[Job], hbm_budget: int = 288 * 10**9, tile_capacity: int = 32768,
                replica_capacity: int = 8192, block: int = 256, slots: int = 2, quantum: int = 64,
                lattice_kind: int = LATTICE_BIT, verify: int = 8, verify_longest: int = 2, seed: int = 0,
-               log=None, spatial_order: bool = True) -> dict:
+               log=None, spatial_order=None) -> dict:
     staged = [W.stage(j.system, lattice_kind, spatial_order=spatial_order) for j in jobs]
     srv = Server(hbm_budget, tile_capacity, replica_capacity, lattice_kind, block, slots, quantum)
     ids: List[Optional[int]] = [None] * len(jobs)

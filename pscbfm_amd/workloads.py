@@ -102,11 +102,12 @@ _FORBIDDEN = None
 
 
 def stage(sd: SystemData, kind: int = LATTICE_BIT, gpu: int = 0, block: int = 0, slots: int = 0,
-          check: bool = False, initialize: bool = False, spatial_order: bool = True) -> UpdaterGPUScBFM_AB_Type:
+          check: bool = False, initialize: bool = False, spatial_order=None) -> UpdaterGPUScBFM_AB_Type:
     """Stages a system in the reference's call order (GPUScBFM_AB_Type.h:83-130);
     initialize() only if asked (a batch takes the staged system as it is).
-    spatial_order: slots of a species in Z-curve order (default) or in the
-    reference's monomer order; results are the same."""
+    spatial_order: slot order of a species, True (Z-curve order), False (the
+    reference's monomer order) or None (the library's automatic choice);
+    results are the same."""
     global _FORBIDDEN
     if _FORBIDDEN is None:
         _FORBIDDEN = standard_forbidden()

@@ -61,10 +61,11 @@ def forbidden_table() -> np.ndarray:
 
 
 def stage_updater(s: Sys, seed: int, kind: int = P.LATTICE_BIT, colors=None, gpu: int = 0, bulk: bool = True,
-                  block: int = 0, slots: int = 0, spatial: bool = True) -> P.UpdaterGPUScBFM_AB_Type:
+                  block: int = 0, slots: int = 0, spatial=None) -> P.UpdaterGPUScBFM_AB_Type:
     """Stages a system in the call order of GPUScBFM_AB_Type.h:83-130, up to
     (not including) initialize.  spatial: slot order inside each species
-    (Z-curve, the default, or monomer order)."""
+    (True: Z-curve order, False: monomer order, None: the library's
+    automatic choice)."""
     u = P.UpdaterGPUScBFM_AB_Type()
     u.setGpu(gpu)
     u.setPeriodicity(True, True, True)
@@ -95,7 +96,7 @@ def stage_updater(s: Sys, seed: int, kind: int = P.LATTICE_BIT, colors=None, gpu
 
 
 def make_updater(s: Sys, seed: int, kind: int = P.LATTICE_BIT, colors=None, gpu: int = 0, bulk: bool = True,
-                 block: int = 0, slots: int = 0, spatial: bool = True) -> P.UpdaterGPUScBFM_AB_Type:
+                 block: int = 0, slots: int = 0, spatial=None) -> P.UpdaterGPUScBFM_AB_Type:
     """Configures and initializes an updater (GPUScBFM_AB_Type.h:83-130)."""
     u = stage_updater(s, seed, kind, colors, gpu, bulk, block, slots, spatial)
     u.initialize()
@@ -108,7 +109,7 @@ def oracle_for(s: Sys, colors, seed: int) -> oracle.System:
 
 
 def gpu_vs_oracle(s: Sys, seed: int, n_mcs: int, kind: int = P.LATTICE_BIT, chunks=(None,), use_graph=True,
-                  block: int = 0, slots: int = 0, spatial: bool = True):
+                  block: int = 0, slots: int = 0, spatial=None):
     """Runs n_mcs sweeps on the GPU (split into `chunks` calls) and on the
     oracle; returns (gpu_xyz, gpu_log, oracle_xyz, oracle_log)."""
     u = make_updater(s, seed, kind, block=block, slots=slots, spatial=spatial)

@@ -89,7 +89,7 @@ void check_slot_order(const SystemLayout& L, bool spatial, const std::string& na
 // Layout of the system (coloring included) with both coloring modes and
 // both slot orders; the per-slot codes and image counters must decode back
 // to the input.
-std::vector<uint8_t> check_layout(const Sys& s, const char* name) {
+std::vector<uint8_t> check_layout(const Sys& s, const std::string& name, bool chainsOnly) {
     SystemInput in;
     std::memcpy(in.box, s.box, sizeof in.box);
     in.xyz = s.xyz;
@@ -97,14 +97,27 @@ std::vector<uint8_t> check_layout(const Sys& s, const char* name) {
     forbidden_table(in.forbidden);
     std::vector<uint8_t> colors;
     for (bool spatial : {false, true}) {
-        in.spatialOrder = spatial;
+        in.slotOrder = spatial ? SlotOrder::Spatial : SlotOrder::Reference;
         const SystemLayout L = build_layout(in, nullptr, true);
+        expect(L.spatialOrder == spatial, name + ": the slot order asked for");
         check_slot_order(L, spatial, name);
         std::vector<int32_t> back(s.xyz.size());
         const int32_t* img[3] = {L.img[0].data(), L.img[1].data(), L.img[2].data()};
         L.unfolded(L.pos.data(), img, back.data());
         expect(back == s.xyz, std::string(name) + ": layout in either slot order decodes to its input");
     }
+    // the automatic order: spatial on the bit lattice; on the byte lattice
+    // only for scattered partners, which chains (partners at adjacent
+    // ranks) never have
+    in.slotOrder = SlotOrder::Automatic;
+    in.latticeKind = kLatticeBit;
+    expect(build_layout(in, nullptr, true).spatialOrder, name + ": automatic order on the bit lattice is spatial");
+    if (chainsOnly) {
+        in.latticeKind = kLatticeU8;
+        expect(!build_layout(in, nullptr, true).spatialOrder,
+               name + ": automatic order of chains on the byte lattice is the reference's");
+    }
+    in.slotOrder = SlotOrder::Spatial;
     for (bool uniform : {true, false}) {
         const SystemLayout L = build_layout(in, nullptr, uniform);
         std::vector<int32_t> back(s.xyz.size());
@@ -180,21 +193,21 @@ int main(int argc, char** argv) {
             expect(grow_linear_chains(s.box, 20, 16, 7, 200, s.xyz), "grown chains");
             for (uint32_t c = 0; c < 20; ++c)
                 for (uint32_t i = 0; i + 1 < 16; ++i) s.bonds.push_back({c * 16 + i, c * 16 + i + 1});
-            check_oracle(s, check_layout(s, "grown chains"), "grown chains");
+            check_oracle(s, check_layout(s, "grown chains", true), "grown chains");
         }
         {
             Sys s{{64, 64, 64}, {}, {}};
             expect(place_linear_chains_ordered(s.box, 512, 32, s.xyz), "ordered chains");
             for (uint32_t c = 0; c < 512; ++c)
                 for (uint32_t i = 0; i + 1 < 32; ++i) s.bonds.push_back({c * 32 + i, c * 32 + i + 1});
-            check_oracle(s, check_layout(s, "ordered chains"), "ordered chains");
+            check_oracle(s, check_layout(s, "ordered chains", true), "ordered chains");
         }
         {
             Sys s{{64, 64, 64}, {}, {}};
             std::vector<uint32_t> pairs;
             expect(build_star_network(s.box, 40, 4, 29, 9, s.xyz, pairs), "star network");
             for (size_t k = 0; k + 1 < pairs.size(); k += 2) s.bonds.push_back({pairs[k], pairs[k + 1]});
-            check_oracle(s, check_layout(s, "star network"), "star network");
+            check_oracle(s, check_layout(s, "star network", false), "star network");
         }
         // non-cubic boxes: the Z-curve then interleaves fewer bits on the
         // short axes, and the slot sort covers codes of fewer bits
@@ -203,21 +216,22 @@ int main(int argc, char** argv) {
             expect(grow_linear_chains(s.box, 12, 24, 11, 200, s.xyz), "grown chains, 16x32x64");
             for (uint32_t c = 0; c < 12; ++c)
                 for (uint32_t i = 0; i + 1 < 24; ++i) s.bonds.push_back({c * 24 + i, c * 24 + i + 1});
-            check_oracle(s, check_layout(s, "grown chains, 16x32x64"), "grown chains, 16x32x64");
+            check_oracle(s, check_layout(s, "grown chains, 16x32x64", true), "grown chains, 16x32x64");
         }
         {
             Sys s{{128, 32, 64}, {}, {}};
             std::vector<uint32_t> pairs;
             expect(build_star_network(s.box, 30, 4, 19, 5, s.xyz, pairs), "star network, 128x32x64");
             for (size_t k = 0; k + 1 < pairs.size(); k += 2) s.bonds.push_back({pairs[k], pairs[k + 1]});
-            check_oracle(s, check_layout(s, "star network, 128x32x64"), "star network, 128x32x64");
+            check_oracle(s, check_layout(s, "star network, 128x32x64", false), "star network, 128x32x64");
         }
         // the reference's fixtures: read, lay out, write and read back
         for (const char* f : {"LinearChain_l1024_n64.bfm.gz", "Ring_N6_n6.bfm", "Hydrogel_NStar_117_NoPerXYZ64.bfm"}) {
             const BfmData b = read_bfm(golden + "/" + f);
             expect(b.nMonomers() > 0, std::string(f) + ": monomers read");
             const Sys s = from_bfm(b);
-            const std::vector<uint8_t> colors = check_layout(s, f);
+            const bool chains = std::string(f).find("Hydrogel") == std::string::npos;  // chains and a ring
+            const std::vector<uint8_t> colors = check_layout(s, f, chains);
             if (b.nMonomers() <= 1000) check_oracle(s, colors, f);
             const std::string out = tmp + "/round_trip.bfm";
             {

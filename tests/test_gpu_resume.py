@@ -17,7 +17,7 @@ pytestmark = pytest.mark.gpu
 EXE = Path(P.__file__).parent / "pscbfm_run"
 
 
-def _resumed(s, seed, start, mcs, colors=None, kind=P.LATTICE_BIT, spatial=True):
+def _resumed(s, seed, start, mcs, colors=None, kind=P.LATTICE_BIT, spatial=None):
     u = stage_updater(s, seed, kind, colors=colors, spatial=spatial)
     u.setStartMcs(start)
     u.initialize()

@@ -520,6 +520,16 @@ def test_perform_walk_is_validated():
             u.setPerformWalk(group, depth)
 
 
+def test_slot_order_is_validated():
+    u = P.UpdaterGPUScBFM_AB_Type()
+    for order in (None, "auto", True, "spatial", False, "reference"):
+        u.setSlotOrder(order)
+    with pytest.raises(P.InvalidArgument):
+        u.setSlotOrder("hilbert")
+    # the C ABI takes 0, 1 or 2 only
+    assert u._lib.pscbfm_set_slot_order(u._h, 3) == 1  # PSCBFM_INVALID_ARGUMENT
+
+
 def test_clear_pass_is_validated():
     u = P.UpdaterGPUScBFM_AB_Type()
     for p in (0, 1, 2):
