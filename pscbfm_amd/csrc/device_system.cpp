@@ -76,7 +76,8 @@ Plan plan(const SystemLayout& L, uint32_t latticeKind, bool withLists, uint32_t 
     const uint64_t nMono = std::max<uint64_t>(L.nMonomers, 1);
     p.bytes[P_SLOTOF] = nMono * 4;
     p.bytes[P_XYZ] = withLists ? nMono * 12 : 4;
-    p.bytes[P_KEY] = nSlots * 4;
+    // draw keys: read only by systems whose slots are in spatial order
+    p.bytes[P_KEY] = L.spatialOrder ? nSlots * 4 : 4;
     p.total = 0;
     for (int i = 0; i < P_N; ++i) {
         p.off[i] = p.total;
@@ -137,7 +138,7 @@ void DeviceSystem::allocate(const SystemLayout& L, uint32_t latticeKind, uint64_
         if (n) hip_check(hipMemcpyAsync(base + p.off[piece], src, n, hipMemcpyHostToDevice, stream), "H2D");
     };
     up(P_POS, L.pos.data(), uint64_t(L.nSlots) * 4);
-    up(P_KEY, L.key.data(), uint64_t(L.nSlots) * 4);
+    if (L.spatialOrder) up(P_KEY, L.key.data(), uint64_t(L.nSlots) * 4);
     up(P_SLOTOF, L.slotOf.data(), uint64_t(L.nMonomers) * 4);
     up(P_IMGX, L.img[0].data(), uint64_t(L.nSlots) * 4);
     up(P_IMGY, L.img[1].data(), uint64_t(L.nSlots) * 4);
