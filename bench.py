@@ -162,23 +162,49 @@ def cpu_single_core(u, box, bonds, colors, seed, target_seconds: float) -> dict:
     return out
 
 
-def network_section(seed: int, mcs: int, kind: int, c_l: float, shape=None, clear_pass: int = 0,
-                    spatial_order=None) -> dict:
-    """Star network: 12800 4-arm stars of arm length 29 (1,497,600 monomers)
-    with arm ends crosslinked to other stars, on a 512x256x256 periodic box
-    (occupied fraction 0.357; the same monomer count on 256^3 would be 0.72,
-    beyond what seeded self-avoiding growth reaches).  Times `mcs` sweeps and
-    checks the first 3 sweeps of a second copy against the oracle."""
+def _oracle_of(sd, colors):
     import oracle
 
-    box = (512, 256, 256)
+    n = sd.n
+    pairs = W.directed_pairs(n, sd.bonds)
+    start = np.concatenate([[0], np.cumsum(np.bincount(pairs[:, 0].astype(np.int64), minlength=n))]).astype(np.uint32)
+    return oracle.System(sd.box, sd.xyz, start, pairs[:, 1].copy(), colors, seed=sd.seed)
+
+
+def network_section(seed: int, mcs: int, kind: int, c_l: float, shape=None, clear_pass: int = 0,
+                    spatial_order=None, dense: bool = True) -> dict:
+    """Star network of 12800 4-arm stars of arm length 29 (1,497,600
+    monomers).  dense: on 256^3 (occupied fraction 0.714), built from an
+    ordered packing of the stars, 100 sweeps of the stars on the GPU (checked
+    against the oracle) and two rounds of arm-end crosslinks, four species.
+    Else grown: stars grown as self-avoiding walks in a 512x256x256 box
+    (occupied fraction 0.357, as dense as growth gets), one round of
+    crosslinks, three species.  Times `mcs` sweeps and checks the first 3
+    sweeps of a second copy against the oracle."""
     tic = time.perf_counter()
-    xyz, bonds = P.gen_star_network(box, 12800, 4, 29, seed)
-    sd = W.SystemData(box, xyz, bonds, seed, "saw")
+    relax_parity = None
+    if dense:
+        stars = W.dense_stars(seed)
+        relax_mcs = W.DENSE_NETWORK["relax_mcs"]
+        moved = W.relax_on_gpu(kind)(stars, relax_mcs)
+        sd = W.crosslinked(stars, moved, W.DENSE_NETWORK["n_stars"], W.DENSE_NETWORK["arm_length"])
+        built = time.perf_counter() - tic
+        # the run that built it, on the oracle
+        v = W.stage(stars, kind, initialize=True)
+        o = _oracle_of(stars, v.colors(stars.n))
+        v.cleanup()
+        o.run(0, relax_mcs * o.n_colors)
+        relax_parity = bool(np.array_equal(o.xyz, moved))
+        log(f"dense network: stars' {relax_mcs}-sweep build run equal to the oracle's: {relax_parity}")
+    else:
+        xyz, bonds = P.gen_star_network((512, 256, 256), 12800, 4, 29, seed)
+        sd = W.SystemData((512, 256, 256), xyz, bonds, seed, "saw")
+        built = time.perf_counter() - tic
+    box = sd.box
     u = W.stage(sd, kind, initialize=True, spatial_order=spatial_order)
-    n = xyz.shape[0]
+    n = sd.n
     log(f"network: {n} monomers, {u.nColors()} species {u.colorSizes().tolist()}, "
-        f"built in {time.perf_counter() - tic:.1f} s")
+        f"built in {built:.1f} s")
     if clear_pass:
         u.setClearPass(clear_pass)
     if shape:
@@ -204,17 +230,21 @@ def network_section(seed: int, mcs: int, kind: int, c_l: float, shape=None, clea
     _, glog = v.run(3, log=True)
     gx = v.getPositions(n)
     v.cleanup()
-    pairs = W.directed_pairs(n, bonds)
-    start = np.concatenate([[0], np.cumsum(np.bincount(pairs[:, 0].astype(np.int64), minlength=n))]).astype(np.uint32)
-    ids = pairs[:, 1].copy()
-    o = oracle.System(box, xyz, start, ids, colors, seed=seed)
+    o = _oracle_of(sd, colors)
     _, olog = o.run(0, 3 * (int(colors.max()) + 1))
     parity = bool(np.array_equal(gx, o.xyz) and np.array_equal(glog, olog))
     log(f"network parity over 3 MCS: {parity}")
-    return {
-        "workload": f"star network: 12800 4-arm stars (arm 29) = {n} monomers, arm ends crosslinked, "
-                    f"512x256x256 periodic, occupied fraction {8 * n / (512 * 256 * 256):.3f}, "
-                    f"{int(colors.max()) + 1} species, seed {seed}",
+    fraction = 8 * n / (box[0] * box[1] * box[2])
+    if dense:
+        what = (f"dense star network: 12800 4-arm stars (arm 29) = {n} monomers on {box[0]}^3 periodic, "
+                f"occupied fraction {fraction:.3f}; ordered packing of the stars, "
+                f"{W.DENSE_NETWORK['relax_mcs']} sweeps, then two rounds of arm-end crosslinks "
+                f"({len(sd.bonds) - 12800 * 4 * 29} crosslinks)")
+    else:
+        what = (f"grown star network: 12800 4-arm stars (arm 29) = {n} monomers, arm ends crosslinked, "
+                f"{box[0]}x{box[1]}x{box[2]} periodic, occupied fraction {fraction:.3f}")
+    out = {
+        "workload": f"{what}, {int(colors.max()) + 1} species, seed {seed}",
         "value": value,
         "unit": "attempted monomer moves/s",
         "mcs": mcs,
@@ -225,6 +255,10 @@ def network_section(seed: int, mcs: int, kind: int, c_l: float, shape=None, clea
         "shape": {"block": block, "slots": slots},
         "parity_vs_oracle": f"{'bit-exact' if parity else 'MISMATCH'} over 3 MCS",
     }
+    if relax_parity is not None:
+        out["build_run_vs_oracle"] = (f"{'bit-exact' if relax_parity else 'MISMATCH'} over "
+                                      f"{W.DENSE_NETWORK['relax_mcs']} MCS of the stars")
+    return out
 
 
 def batched_vs_plain(n_rep: int, mcs: int, seed: int, kind: int, tune_sample: int, tune_mcs: int,
@@ -391,6 +425,9 @@ def main() -> int:
     ap.add_argument("--batch-shape-after-tune", default="",
                     help="BLOCKxSLOTS the batch uses after the autotuner has run (default: the tuner's pick)")
     ap.add_argument("--no-network", action="store_true")
+    ap.add_argument("--network", choices=("both", "dense", "grown"), default="both",
+                    help="star networks to run: dense (256^3, occupied fraction 0.714) and/or grown "
+                         "(512x256x256, 0.357)")
     ap.add_argument("--compare-mcs", type=int, default=300, help="MCS of the other lattice kind on the same melt")
     ap.add_argument("--compare-cpu-seconds", type=float, default=3.0,
                     help="target seconds of the oracle sample that checks the other lattice kind")
@@ -627,8 +664,12 @@ def main() -> int:
     u.cleanup()
     if not args.no_network and args.network_mcs > 0:
         nshape = tuple(int(v) for v in args.network_shape.split("x")) if args.network_shape else None
-        out["network"] = network_section(args.seed, args.network_mcs, kind, c_l, nshape, args.clear_pass,
-                                         SLOT_ORDER[args.slot_order])
+        if args.network in ("both", "dense"):
+            out["network"] = network_section(args.seed, args.network_mcs, kind, c_l, nshape, args.clear_pass,
+                                             SLOT_ORDER[args.slot_order], dense=True)
+        if args.network in ("both", "grown"):
+            out["network_grown"] = network_section(args.seed, args.network_mcs, kind, c_l, nshape,
+                                                   args.clear_pass, SLOT_ORDER[args.slot_order], dense=False)
     if not args.no_batch and args.replicas > 0:
         out["batched"] = batched_vs_plain(args.replicas, args.replica_mcs, args.seed, kind, args.tune_sample,
                                           args.tune_mcs, plain=not args.no_plain,

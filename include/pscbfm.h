@@ -281,6 +281,28 @@ int pscbfm_gen_linear_chains_ordered(uint32_t lx, uint32_t ly, uint32_t lz, uint
 int pscbfm_gen_star_network(uint32_t lx, uint32_t ly, uint32_t lz, uint32_t n_stars, uint32_t arms,
                             uint32_t arm_length, uint64_t seed, int32_t* xyz, uint32_t* bonds,
                             uint64_t bond_capacity, uint64_t* n_bonds);
+/* Four-arm stars on a dense ordered packing, for occupied fractions that
+ * growth cannot reach: n_stars stars of arm_length monomers (center first,
+ * then the arms) with corners on the sites of even coordinates, n_stars of
+ * the box's star places chosen at random by seed.  Up to 91% of the sites
+ * hold a monomer (arm 29 on 256^3: 16,384 stars).  xyz receives
+ * 3*n_stars*(1+4*arm_length) corners, bonds the 2*n_stars*4*arm_length ids
+ * of the bonds inside the stars; PSCBFM_RUNTIME_ERROR if the box holds fewer
+ * stars.  No crosslinks: every bond between two such sites joins sites of
+ * opposite parity, so a network made there has two species; crosslinks
+ * added after a run (pscbfm_crosslink_star_ends) close odd cycles. */
+int pscbfm_gen_stars_ordered(uint32_t lx, uint32_t ly, uint32_t lz, uint32_t n_stars, uint32_t arm_length,
+                             uint64_t seed, int32_t* xyz, uint32_t* bonds);
+/* Crosslinks for stars at given corners (n_stars stars of 1 + arms *
+ * arm_length monomers, center first, arm by arm; unfolded coordinates, as
+ * after a run).  In each of `rounds` rounds every arm end is bonded to the
+ * first monomer, in a random order of the 108 bond vectors, of another
+ * star's arm interior that has no crosslink of that round yet and is not
+ * already its partner.  bonds receives only the crosslink pairs (capacity in
+ * pairs: n_stars * arms * rounds). */
+int pscbfm_crosslink_star_ends(uint32_t lx, uint32_t ly, uint32_t lz, uint32_t n_stars, uint32_t arms,
+                               uint32_t arm_length, uint32_t rounds, uint64_t seed, const int32_t* xyz,
+                               uint32_t* bonds, uint64_t bond_capacity, uint64_t* n_bonds);
 
 /* Streaming-copy reference on `gpu`: copies `bytes` (read + write) `repeats`
  * times with a plain 16-byte-per-lane kernel and returns the mean time.  Used

@@ -24,7 +24,7 @@ __all__ = [
     "UpdaterGPUScBFM_AB_Type", "InvalidArgument", "DeviceError", "RunStats", "lib", "library_path",
     "LATTICE_U8", "LATTICE_BIT", "standard_bond_vectors", "move_hash", "substep_draw",
     "zcurve_encode", "color_graph", "gen_linear_chains", "gen_linear_chains_ordered",
-    "gen_star_network", "chain_bonds", "Batch", "ReplicaInfo",
+    "gen_star_network", "gen_stars_ordered", "crosslink_star_ends", "chain_bonds", "Batch", "ReplicaInfo",
     "autotune_batch", "TUNE_CANDIDATES", "Server", "JobInfo", "ServerCounters",
 ]
 
@@ -159,6 +159,8 @@ _SIGNATURES = {
     "pscbfm_gen_linear_chains": (C.c_int, [C.c_uint32] * 5 + [C.c_uint64, C.c_uint32, _i32p]),
     "pscbfm_gen_linear_chains_ordered": (C.c_int, [C.c_uint32] * 5 + [_i32p]),
     "pscbfm_gen_star_network": (C.c_int, [C.c_uint32] * 6 + [C.c_uint64, _i32p, _u32p, C.c_uint64, _u64p]),
+    "pscbfm_gen_stars_ordered": (C.c_int, [C.c_uint32] * 5 + [C.c_uint64, _i32p, _u32p]),
+    "pscbfm_crosslink_star_ends": (C.c_int, [C.c_uint32] * 7 + [C.c_uint64, _i32p, _u32p, C.c_uint64, _u64p]),
     "pscbfm_copy_probe": (C.c_int, [C.c_int, C.c_uint64, C.c_int, C.POINTER(C.c_double)]),
     "pscbfm_batch_create": (C.c_int, [C.c_int, C.c_int, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
                                        C.POINTER(_H)]),
@@ -761,3 +763,36 @@ def gen_star_network(box, n_stars: int, arms: int, arm_length: int, seed: int):
                                          _ptr(bonds, C.c_uint32), int(cap), C.byref(nb))
     _raise(code, "star network growth failed")
     return xyz, bonds[: nb.value].copy()
+
+
+def gen_stars_ordered(box, n_stars: int, arm_length: int, seed: int):
+    """Four-arm stars on a dense ordered packing (corners at even
+    coordinates, up to 91% of those sites filled) and their bonds; no
+    crosslinks (see crosslink_star_ends)."""
+    per = 1 + 4 * arm_length
+    xyz = np.empty((n_stars * per, 3), dtype=np.int32)
+    bonds = np.empty((max(n_stars * 4 * arm_length, 1), 2), dtype=np.uint32)
+    code = lib().pscbfm_gen_stars_ordered(int(box[0]), int(box[1]), int(box[2]), int(n_stars), int(arm_length),
+                                          int(seed) & 0xFFFFFFFFFFFFFFFF, _ptr(xyz, C.c_int32),
+                                          _ptr(bonds, C.c_uint32))
+    _raise(code, "ordered stars do not fit the box")
+    return xyz, bonds[: n_stars * 4 * arm_length].copy()
+
+
+def crosslink_star_ends(box, xyz: np.ndarray, n_stars: int, arms: int, arm_length: int, seed: int,
+                        rounds: int = 1) -> np.ndarray:
+    """Crosslink pairs (k, 2) joining arm ends to other stars' arm interiors
+    at valid bond vectors, for stars at the corners xyz (unfolded); each
+    round gives every arm end at most one more partner."""
+    pts = np.ascontiguousarray(xyz, dtype=np.int32)
+    if pts.shape != (n_stars * (1 + arms * arm_length), 3):
+        raise InvalidArgument("corners do not match the star count and size")
+    cap = max(n_stars * arms * int(rounds), 1)
+    bonds = np.empty((cap, 2), dtype=np.uint32)
+    nb = C.c_uint64()
+    code = lib().pscbfm_crosslink_star_ends(int(box[0]), int(box[1]), int(box[2]), int(n_stars), int(arms),
+                                            int(arm_length), int(rounds), int(seed) & 0xFFFFFFFFFFFFFFFF,
+                                            _ptr(pts, C.c_int32),
+                                            _ptr(bonds, C.c_uint32), int(cap), C.byref(nb))
+    _raise(code, "crosslinking failed")
+    return bonds[: nb.value].copy()

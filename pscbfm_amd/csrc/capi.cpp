@@ -564,6 +564,42 @@ int pscbfm_gen_star_network(uint32_t lx, uint32_t ly, uint32_t lz, uint32_t n_st
     }
 }
 
+int pscbfm_gen_stars_ordered(uint32_t lx, uint32_t ly, uint32_t lz, uint32_t n_stars, uint32_t arm_length,
+                             uint64_t seed, int32_t* xyz, uint32_t* bonds) {
+    if (!valid_box(lx, ly, lz) || (n_stars && (!xyz || !bonds))) return PSCBFM_INVALID_ARGUMENT;
+    try {
+        const uint32_t box[3] = {lx, ly, lz};
+        std::vector<int32_t> out;
+        std::vector<uint32_t> b;
+        if (!pscbfm::place_stars_ordered(box, n_stars, arm_length, seed, out, b)) return PSCBFM_RUNTIME_ERROR;
+        if (!out.empty()) std::memcpy(xyz, out.data(), out.size() * sizeof(int32_t));
+        if (!b.empty()) std::memcpy(bonds, b.data(), b.size() * sizeof(uint32_t));
+        return PSCBFM_OK;
+    } catch (...) {
+        return PSCBFM_INTERNAL_ERROR;
+    }
+}
+
+int pscbfm_crosslink_star_ends(uint32_t lx, uint32_t ly, uint32_t lz, uint32_t n_stars, uint32_t arms,
+                               uint32_t arm_length, uint32_t rounds, uint64_t seed, const int32_t* xyz,
+                               uint32_t* bonds, uint64_t bond_capacity, uint64_t* n_bonds) {
+    if (!valid_box(lx, ly, lz) || !n_bonds || (n_stars && (!xyz || !bonds))) return PSCBFM_INVALID_ARGUMENT;
+    try {
+        const uint32_t box[3] = {lx, ly, lz};
+        const size_t n = size_t(n_stars) * (1 + size_t(arms) * arm_length);
+        const std::vector<int32_t> in(xyz, xyz + 3 * n);
+        std::vector<uint32_t> b;
+        pscbfm::HostRng rng(seed);
+        pscbfm::crosslink_arm_ends(box, in, n_stars, arms, arm_length, rounds, rng, b);
+        if (b.size() / 2 > bond_capacity) return PSCBFM_OUT_OF_RANGE;
+        if (!b.empty()) std::memcpy(bonds, b.data(), b.size() * sizeof(uint32_t));
+        *n_bonds = b.size() / 2;
+        return PSCBFM_OK;
+    } catch (...) {
+        return PSCBFM_INTERNAL_ERROR;
+    }
+}
+
 int pscbfm_copy_probe(int gpu, uint64_t bytes, int repeats, double* ms_per_copy) {
     if (bytes < 16 || repeats < 1 || !ms_per_copy) return PSCBFM_INVALID_ARGUMENT;
     void* src = nullptr;

@@ -1,4 +1,5 @@
 // Seeded builders of synthetic scBFM systems (see generators.h).
+#include <algorithm>
 #include <array>
 #include <cstdlib>
 #include <stdexcept>
@@ -155,6 +156,60 @@ bool place_linear_chains_ordered(const uint32_t box[3], uint32_t nChains, uint32
     return true;
 }
 
+void crosslink_arm_ends(const uint32_t box[3], const std::vector<int32_t>& xyz, uint32_t nStars, uint32_t arms,
+                        uint32_t armLength, uint32_t rounds, HostRng& rng, std::vector<uint32_t>& bonds) {
+    // In each round every arm end takes the first (in random vector order)
+    // monomer of another star's arm interior that sits at a valid bond
+    // vector, has no crosslink of this round yet and is not already its
+    // partner.
+    const uint32_t perStar = 1 + arms * armLength;
+    const uint32_t n = nStars * perStar;
+    if (xyz.size() < 3 * size_t(n)) throw std::invalid_argument("crosslink_arm_ends: fewer corners than monomers");
+    if (armLength < 2 || rounds == 0) return;
+    std::vector<int32_t> owner(size_t(box[0]) * box[1] * box[2], -1);
+    auto cellOf = [&](const int32_t* p) {
+        return (uint32_t(p[0]) & (box[0] - 1)) +
+               size_t(box[0]) * ((uint32_t(p[1]) & (box[1] - 1)) + size_t(box[1]) * (uint32_t(p[2]) & (box[2] - 1)));
+    };
+    for (uint32_t i = 0; i < n; ++i) owner[cellOf(xyz.data() + 3 * size_t(i))] = int32_t(i);
+    std::vector<uint8_t> linked(n);
+    std::vector<uint32_t> partners(size_t(nStars) * arms * rounds, ~0u);  // per arm end, one per round
+    const auto& vecs = standard_bond_vectors();
+    for (uint32_t round = 0; round < rounds; ++round) {
+        std::fill(linked.begin(), linked.end(), uint8_t(0));
+        for (uint32_t s = 0; s < nStars; ++s)
+            for (uint32_t a = 0; a < arms; ++a) {
+                const uint32_t end = s * perStar + 1 + a * armLength + (armLength - 1);
+                if (linked[end]) continue;
+                uint32_t* mine = partners.data() + (size_t(s) * arms + a) * rounds;
+                const int32_t* pe = xyz.data() + 3 * size_t(end);
+                uint32_t order[108];
+                for (uint32_t i = 0; i < 108; ++i) order[i] = i;
+                for (uint32_t i = 108; i > 1; --i) std::swap(order[i - 1], order[rng.below(i)]);
+                for (uint32_t v = 0; v < 108; ++v) {
+                    const auto& b = vecs[order[v]];
+                    const int32_t c[3] = {pe[0] + b[0], pe[1] + b[1], pe[2] + b[2]};
+                    const int32_t o = owner[cellOf(c)];
+                    if (o < 0) continue;
+                    const uint32_t os = uint32_t(o) / perStar, ok = uint32_t(o) % perStar;
+                    const bool interior = ok != 0 && ((ok - 1) % armLength) != armLength - 1;
+                    if (os == s || !interior || linked[o]) continue;
+                    if (std::find(mine, mine + round, uint32_t(o)) != mine + round) continue;
+                    linked[end] = 1;
+                    linked[o] = 1;
+                    mine[round] = uint32_t(o);
+                    bonds.push_back(end);
+                    bonds.push_back(uint32_t(o));
+                    break;
+                }
+            }
+    }
+    // A crosslink found through the periodic box can join stars whose
+    // unfolded coordinates differ by whole box lengths; such a bond is valid
+    // under the minimum-image convention (the kernels only use the bond
+    // vector modulo 8, a divisor of every box edge).
+}
+
 bool build_star_network(const uint32_t box[3], uint32_t nStars, uint32_t arms, uint32_t armLength,
                         uint64_t seed, std::vector<int32_t>& xyz, std::vector<uint32_t>& bonds) {
     const uint32_t perStar = 1 + arms * armLength;
@@ -195,45 +250,88 @@ bool build_star_network(const uint32_t box[3], uint32_t nStars, uint32_t arms, u
                 bonds.push_back(s * perStar + idx);
             }
     }
-    // Crosslinks: each arm end takes the first (in random vector order)
-    // monomer of another star's arm interior that sits at a valid bond vector
-    // and has no crosslink yet.
-    const uint32_t n = nStars * perStar;
-    std::vector<int32_t> owner(size_t(box[0]) * box[1] * box[2], -1);
-    auto cellOf = [&](const int32_t* p) {
-        return (uint32_t(p[0]) & (box[0] - 1)) +
-               size_t(box[0]) * ((uint32_t(p[1]) & (box[1] - 1)) + size_t(box[1]) * (uint32_t(p[2]) & (box[2] - 1)));
+    crosslink_arm_ends(box, xyz, nStars, arms, armLength, 1, rng, bonds);
+    return true;
+}
+
+bool place_stars_ordered(const uint32_t box[3], uint32_t nStars, uint32_t armLength, uint64_t seed,
+                         std::vector<int32_t>& xyz, std::vector<uint32_t>& bonds) {
+    const uint32_t a = armLength;
+    const uint32_t nx = box[0] / 2, ny = box[1] / 2, nz = box[2] / 2;  // sites per axis (powers of two)
+    xyz.clear();
+    bonds.clear();
+    if (nStars == 0) return true;
+    if (a == 0) return false;
+    // Centers P sites apart along a row, P the largest power-of-two share of
+    // the row that holds a star (2a + 1 sites).
+    uint32_t perRow = 1;
+    while (perRow * 2 <= nx && nx / (perRow * 2) >= 2 * a + 1) perRow *= 2;
+    const uint32_t P = nx / perRow;
+    if (P < 2 * a + 1) return false;
+    // Row (y, z) has its centers at offset(y) + j P.  An odd row holds the
+    // third arms [c, c + a - 1] of row (y - 1, z) and the fourth arms
+    // [c' - a + 1, c'] of row (y, z - 1), whose centers are shifted by delta:
+    // the two sets are disjoint for 2a - 1 <= delta <= P - 1, and
+    // offset(y) = delta * y mod P wraps consistently when delta * ny is a
+    // multiple of P.
+    const uint32_t step = P / std::min(P, ny);  // delta must be a multiple of this
+    const uint32_t lo = 2 * a - 1, hi = P - 1, mid = (lo + hi) / 2;
+    uint32_t delta = 0;
+    auto dist = [&](uint32_t d) { return d > mid ? d - mid : mid - d; };
+    for (uint32_t d = step; d <= hi; d += step)
+        if (d >= lo && (delta == 0 || dist(d) < dist(delta))) delta = d;
+    if (delta == 0) return false;
+
+    const uint64_t places = uint64_t(perRow) * ny * nz / 2;
+    if (nStars > places) return false;
+    // nStars places at random (partial Fisher-Yates), filled in row order.
+    HostRng rng(seed);
+    std::vector<uint32_t> pick(places);
+    for (uint64_t i = 0; i < places; ++i) pick[i] = uint32_t(i);
+    for (uint32_t i = 0; i < nStars; ++i) {
+        const uint32_t r = i + uint32_t((uint64_t(uint32_t(rng.next() >> 32)) * (places - i)) >> 32);
+        std::swap(pick[i], pick[r]);
+    }
+    pick.resize(nStars);
+    std::sort(pick.begin(), pick.end());
+
+    const uint32_t perStar = 1 + 4 * a;
+    xyz.assign(3 * size_t(nStars) * perStar, 0);
+    std::vector<uint8_t> taken(size_t(nx) * ny * nz, 0);
+    auto site = [&](int64_t x, int64_t y, int64_t z) {
+        const uint64_t X = uint64_t(x) & (nx - 1), Y = uint64_t(y) & (ny - 1), Z = uint64_t(z) & (nz - 1);
+        return size_t(X + nx * (Y + uint64_t(ny) * Z));
     };
-    for (uint32_t i = 0; i < n; ++i) owner[cellOf(xyz.data() + 3 * size_t(i))] = int32_t(i);
-    std::vector<uint8_t> linked(n, 0);
-    const auto& vecs = standard_bond_vectors();
-    for (uint32_t s = 0; s < nStars; ++s)
-        for (uint32_t a = 0; a < arms; ++a) {
-            const uint32_t end = s * perStar + 1 + a * armLength + (armLength - 1);
-            if (linked[end] || armLength < 2) continue;
-            const int32_t* pe = xyz.data() + 3 * size_t(end);
-            uint32_t order[108];
-            for (uint32_t i = 0; i < 108; ++i) order[i] = i;
-            for (uint32_t i = 108; i > 1; --i) std::swap(order[i - 1], order[rng.below(i)]);
-            for (uint32_t v = 0; v < 108; ++v) {
-                const auto& b = vecs[order[v]];
-                const int32_t c[3] = {pe[0] + b[0], pe[1] + b[1], pe[2] + b[2]};
-                const int32_t o = owner[cellOf(c)];
-                if (o < 0) continue;
-                const uint32_t os = uint32_t(o) / perStar, ok = uint32_t(o) % perStar;
-                const bool interior = ok != 0 && ((ok - 1) % armLength) != armLength - 1;
-                if (os == s || !interior || linked[o]) continue;
-                linked[end] = 1;
-                linked[o] = 1;
-                bonds.push_back(end);
-                bonds.push_back(uint32_t(o));
-                break;
-            }
+    for (uint32_t s = 0; s < nStars; ++s) {
+        // place -> (row, j): rows with y + z even in z-major order, perRow places each
+        const uint32_t row = pick[s] / perRow, j = pick[s] % perRow;
+        const uint32_t z = row / (ny / 2);
+        const uint32_t y = 2 * (row % (ny / 2)) + (z & 1u);
+        const int64_t c = (uint64_t(delta) * y % P) + uint64_t(j) * P;
+        int32_t* star = xyz.data() + 3 * size_t(s) * perStar;
+        auto put = [&](uint32_t idx, int64_t sx, int64_t sy, int64_t sz) {
+            int32_t* p = star + 3 * size_t(idx);
+            p[0] = int32_t(2 * sx);
+            p[1] = int32_t(2 * sy);
+            p[2] = int32_t(2 * sz);
+            uint8_t& t = taken[site(sx, sy, sz)];
+            if (t) throw std::logic_error("ordered stars: two monomers on one site");
+            t = 1;
+        };
+        put(0, c, y, z);
+        for (uint32_t k = 0; k < a; ++k) {
+            put(1 + k, c + 1 + k, y, z);                            // arm 1: +x
+            put(1 + a + k, c - 1 - int64_t(k), y, z);               // arm 2: -x
+            put(1 + 2 * a + k, c + k, int64_t(y) + 1, z);           // arm 3: up in y, then +x
+            put(1 + 3 * a + k, c - int64_t(k), y, int64_t(z) + 1);  // arm 4: up in z, then -x
         }
-    // A crosslink found through the periodic box can join stars whose
-    // unfolded coordinates differ by whole box lengths; such a bond is valid
-    // under the minimum-image convention (the kernels only use the bond
-    // vector modulo 8, a divisor of every box edge).
+        for (uint32_t arm = 0; arm < 4; ++arm)
+            for (uint32_t k = 0; k < a; ++k) {
+                const uint32_t idx = 1 + arm * a + k;
+                bonds.push_back(s * perStar + (k == 0 ? 0 : idx - 1));
+                bonds.push_back(s * perStar + idx);
+            }
+    }
     return true;
 }
 

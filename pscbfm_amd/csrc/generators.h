@@ -38,11 +38,36 @@ bool grow_linear_chains(const uint32_t box[3], uint32_t nChains, uint32_t len, u
 bool place_linear_chains_ordered(const uint32_t box[3], uint32_t nChains, uint32_t len,
                                  std::vector<int32_t>& xyz);
 
+// Crosslinks of a star system (stars of 1 + arms * armLength monomers,
+// center first, arm by arm): in each of `rounds` rounds every arm end is
+// bonded to the first monomer, in a random order of the 108 bond vectors, of
+// another star's arm interior that has no crosslink of that round yet and is
+// not already its partner.  Appends the pairs to bonds.
+void crosslink_arm_ends(const uint32_t box[3], const std::vector<int32_t>& xyz, uint32_t nStars, uint32_t arms,
+                        uint32_t armLength, uint32_t rounds, HostRng& rng, std::vector<uint32_t>& bonds);
+
 // Star network: stars of `arms` arms of `armLength` monomers around one
-// center (monomer 0 of each star), placed on an ordered lattice, then each
-// arm end bonded to a free monomer of another star's arm within a valid bond
-// vector when one exists.  bonds receives undirected pairs (a<b listed once).
+// center (monomer 0 of each star), each star grown as self-avoiding walks
+// from a random free center, then each arm end bonded to a free monomer of
+// another star's arm interior within a valid bond vector when one exists.
+// bonds receives undirected pairs.
 bool build_star_network(const uint32_t box[3], uint32_t nStars, uint32_t arms, uint32_t armLength,
                         uint64_t seed, std::vector<int32_t>& xyz, std::vector<uint32_t>& bonds);
+
+// Dense ordered packing of four-arm stars, for occupied fractions that
+// growth cannot reach: corners on the sublattice of even coordinates
+// (sites).  A star's center and its +x and -x arms lie on one row of sites
+// along x; its third arm starts one site up in y and runs along +x, its
+// fourth starts one site up in z and runs along -x.  Rows (y, z) with y + z
+// even carry centers, P sites apart; every other row carries the third arms
+// of the row below it in y and the fourth arms of the row below it in z,
+// the two sets interleaved by shifting the centers of neighbouring rows.
+// nStars of the available star places, chosen at random, are filled, in
+// row order; bonds receives the bonds inside the stars (no crosslinks: every
+// bond between sites joins opposite parities, so crosslinks are added after
+// a run has moved the corners off the sublattice).  Returns false if the box
+// holds fewer than nStars stars.
+bool place_stars_ordered(const uint32_t box[3], uint32_t nStars, uint32_t armLength, uint64_t seed,
+                         std::vector<int32_t>& xyz, std::vector<uint32_t>& bonds);
 
 }  // namespace pscbfm

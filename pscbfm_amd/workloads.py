@@ -6,6 +6,15 @@ fraction phi uniform in [0.1, 0.5], round(phi * V / (8 N)) chains (at least
 one), a per-replica seed.  Chains are grown as seeded self-avoiding walks
 with bond vectors drawn uniformly from the 108 set; a box too dense for the
 growth falls back to the ordered serpentine packing (recorded per replica).
+
+Dense star network: 12,800 four-arm stars of arm 29 (1,497,600 monomers)
+on 256^3, occupied fraction 0.714, beyond what seeded growth reaches.  The
+stars start on an ordered packing of the sites with even coordinates, run
+100 sweeps as a star system, and then get two rounds of crosslinks from
+arm ends to other stars' arm interiors at valid bond vectors.  On the
+packing every bond joins sites of opposite parity, so the stars alone have
+two species; the crosslinks made after the run close odd cycles, and the
+reference's coloring splits the network into four species.
 """
 from __future__ import annotations
 
@@ -16,10 +25,13 @@ from typing import List, Optional
 
 import numpy as np
 
-from . import (LATTICE_BIT, UpdaterGPUScBFM_AB_Type, chain_bonds, gen_linear_chains, gen_linear_chains_ordered,
-               standard_bond_vectors)
+from . import (LATTICE_BIT, UpdaterGPUScBFM_AB_Type, chain_bonds, crosslink_star_ends, gen_linear_chains,
+               gen_linear_chains_ordered, gen_stars_ordered, standard_bond_vectors)
 
 EDGES = (32, 64, 128)
+
+# The dense star network's build parameters.
+DENSE_NETWORK = {"box": (256, 256, 256), "n_stars": 12800, "arm_length": 29, "relax_mcs": 100, "rounds": 2}
 
 
 @dataclass
@@ -77,6 +89,43 @@ def build_replicas(specs: List[ReplicaSpec], workers: int = 0) -> List[SystemDat
     workers = workers or min(16, os.cpu_count() or 1)
     with cf.ThreadPoolExecutor(max_workers=workers) as ex:
         return list(ex.map(build_chains, specs))
+
+
+def dense_stars(seed: int, box=DENSE_NETWORK["box"], n_stars: int = DENSE_NETWORK["n_stars"],
+                arm_length: int = DENSE_NETWORK["arm_length"]) -> SystemData:
+    """The dense network's stars on their ordered packing, without crosslinks."""
+    xyz, bonds = gen_stars_ordered(box, n_stars, arm_length, seed)
+    return SystemData(tuple(box), xyz, bonds, seed, "ordered")
+
+
+def crosslinked(stars: SystemData, moved: np.ndarray, n_stars: int, arm_length: int,
+                rounds: int = DENSE_NETWORK["rounds"]) -> SystemData:
+    """The stars at the corners `moved` (a run of `stars`), with `rounds`
+    rounds of arm-end crosslinks made at those corners."""
+    moved = np.ascontiguousarray(moved, dtype=np.int32)
+    cross = crosslink_star_ends(stars.box, moved, n_stars, 4, arm_length, stars.seed, rounds)
+    return SystemData(stars.box, moved, np.concatenate([stars.bonds, cross]), stars.seed, "ordered+run")
+
+
+def dense_network(seed: int, relax, box=DENSE_NETWORK["box"], n_stars: int = DENSE_NETWORK["n_stars"],
+                  arm_length: int = DENSE_NETWORK["arm_length"], relax_mcs: int = DENSE_NETWORK["relax_mcs"],
+                  rounds: int = DENSE_NETWORK["rounds"]) -> SystemData:
+    """The dense star network.  relax(system, mcs) runs the star system and
+    returns its unfolded corners (relax_on_gpu, or any bit-exact runner)."""
+    stars = dense_stars(seed, box, n_stars, arm_length)
+    return crosslinked(stars, relax(stars, relax_mcs), n_stars, arm_length, rounds)
+
+
+def relax_on_gpu(kind: int = LATTICE_BIT, gpu: int = 0):
+    """A relax(system, mcs) that runs the system in an updater on `gpu`."""
+    def run(sd: SystemData, mcs: int) -> np.ndarray:
+        u = stage(sd, kind, gpu=gpu, initialize=True)
+        try:
+            u.runSimulationOnGPU(int(mcs))
+            return u.getPositions(sd.n)
+        finally:
+            u.cleanup()
+    return run
 
 
 def standard_forbidden() -> np.ndarray:

@@ -56,6 +56,17 @@ def stars(box, n_stars, arms, arm_length, seed) -> Sys:
     return Sys(tuple(box), xyz, bonds)
 
 
+def oracle_relax(sd, mcs: int) -> np.ndarray:
+    """A relax(system, mcs) for the workload builders on the CPU oracle, with
+    the coloring the updater computes; returns the unfolded corners."""
+    s = Sys(tuple(sd.box), sd.xyz, sd.bonds)
+    start, ids = s.csr()
+    colors = P.color_graph(s.n, start, ids)
+    o = oracle.System(s.box, s.xyz, start, ids, colors, forbidden_table(), sd.seed)
+    o.run(0, int(mcs) * (int(colors.max()) + 1))
+    return o.xyz
+
+
 def forbidden_table() -> np.ndarray:
     return oracle.standard_bondset()
 

@@ -20,7 +20,8 @@ import pytest
 import oracle
 import pscbfm_amd as P
 from pscbfm_amd.bfm import read_bfm, write_bfm
-from helpers import GOLDEN, Sys, chains, oracle_for, ordered_chains, stars
+import pscbfm_amd.workloads as W
+from helpers import GOLDEN, Sys, chains, oracle_for, oracle_relax, ordered_chains, stars
 
 ROOT = Path(__file__).resolve().parents[1]
 
@@ -335,6 +336,64 @@ def test_star_network_degrees():
     deg = np.diff(start.astype(np.int64))
     assert deg.max() <= 4 and deg[0] == 4
     assert s.bonds.shape[0] > 20 * 4 * 29   # some crosslinks formed
+
+
+def test_dense_stars_fill_the_ordered_packing():
+    """Stars at occupied fraction 0.714 on 256^3: a valid state (no overlap,
+    every bond in the set), all bonds inside stars, two species (every bond
+    of the packing joins sites of opposite parity); seeded; a box that holds
+    fewer stars is refused."""
+    sd = W.dense_stars(42)
+    assert sd.n == 12800 * 117 and 8 * sd.n / 256 ** 3 == pytest.approx(0.714, abs=1e-3)
+    assert sd.bonds.shape == (12800 * 4 * 29, 2)
+    assert np.all(sd.bonds[:, 0] // 117 == sd.bonds[:, 1] // 117)
+    s = Sys(sd.box, sd.xyz, sd.bonds)
+    start, ids = s.csr()
+    colors = P.color_graph(s.n, start, ids)
+    assert int(colors.max()) + 1 == 2
+    assert oracle_for(s, colors, 0).check() == 0
+    assert np.array_equal(sd.xyz, W.dense_stars(42).xyz) and not np.array_equal(sd.xyz, W.dense_stars(43).xyz)
+    assert W.dense_stars(1, (256, 256, 256), 16384).n == 16384 * 117   # every place filled: 91% of the sites
+    with pytest.raises(RuntimeError):
+        W.dense_stars(1, (256, 256, 256), 16385)
+    for box, n_stars, arm in [((32, 32, 32), 40, 5), ((64, 16, 32), 20, 3), ((16, 64, 16), 10, 2), ((8, 8, 8), 4, 1)]:
+        t = W.dense_stars(3, box, n_stars, arm)
+        u = Sys(t.box, t.xyz, t.bonds)
+        st, ids = u.csr()
+        assert oracle_for(u, P.color_graph(u.n, st, ids), 0).check() == 0
+
+
+def test_dense_network_after_a_run_has_four_species():
+    """The dense network as built for the bench: 100 sweeps of the stars on
+    the oracle, then two rounds of arm-end crosslinks.  Valid state, at most
+    two crosslinks per arm end, no bond twice, crosslinks only between
+    stars, and the reference's coloring gives four species."""
+    sd = W.dense_network(42, oracle_relax)
+    n_star_bonds = 12800 * 4 * 29
+    cross = sd.bonds[n_star_bonds:]
+    assert len(cross) > 1.5 * 12800 * 4     # most arm ends linked in both rounds
+    assert np.all(cross[:, 0] // 117 != cross[:, 1] // 117)
+    key = np.sort(sd.bonds, axis=1)
+    assert np.unique(key, axis=0).shape[0] == key.shape[0]
+    assert np.bincount(cross[:, 0]).max() <= 2
+    s = Sys(sd.box, sd.xyz, sd.bonds)
+    start, ids = s.csr()
+    assert np.diff(start.astype(np.int64)).max() <= 4
+    colors = P.color_graph(s.n, start, ids)
+    assert int(colors.max()) + 1 == 4
+    assert oracle_for(s, colors, 0).check() == 0
+
+
+def test_crosslink_rounds_and_arguments():
+    sd = W.dense_stars(5, (64, 64, 64), 120, 8)
+    moved = oracle_relax(sd, 30)
+    one = P.crosslink_star_ends(sd.box, moved, 120, 4, 8, seed=5, rounds=1)
+    two = P.crosslink_star_ends(sd.box, moved, 120, 4, 8, seed=5, rounds=2)
+    assert len(P.crosslink_star_ends(sd.box, moved, 120, 4, 8, seed=5, rounds=0)) == 0
+    assert np.array_equal(two[: len(one)], one)           # round one is the same either way
+    assert np.bincount(one[:, 0]).max() == 1 and np.bincount(two[:, 0]).max() <= 2 and len(two) > len(one)
+    with pytest.raises(P.InvalidArgument):
+        P.crosslink_star_ends(sd.box, moved[:-1], 120, 4, 8, seed=5)
 
 
 def test_native_bfm_reader_and_writer_match_python(tmp_path):
