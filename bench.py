@@ -172,7 +172,7 @@ def _oracle_of(sd, colors):
 
 
 def network_section(seed: int, mcs: int, kind: int, c_l: float, shape=None, clear_pass: int = 0,
-                    spatial_order=None, dense: bool = True) -> dict:
+                    spatial_order=None, dense: bool = True, after_tune=None) -> dict:
     """Star network of 12800 4-arm stars of arm length 29 (1,497,600
     monomers).  dense: on 256^3 (occupied fraction 0.714), built from an
     ordered packing of the stars, 100 sweeps of the stars on the GPU (checked
@@ -216,6 +216,12 @@ def network_section(seed: int, mcs: int, kind: int, c_l: float, shape=None, clea
         block, slots, trials = u.autotune(3)
         log(f"network autotune: block {block} slots {slots}; "
             + ", ".join(f"{b}x{s_}:{ms:.3f}" for (b, s_), ms in sorted(trials.items())))
+        if after_tune:
+            # the tuner ran as usual; the network takes the stated shape
+            # (profiled runs: the tuner's choice drifts under a profiler)
+            block, slots = after_tune
+            u.setLaunchShape(block, slots)
+            log(f"network shape after tuning fixed at block {block} slots {slots}")
     u.run(20)
     st, _ = u.run(mcs)
     p1, p2 = algorithmic_bytes(st, c_l)
@@ -422,6 +428,8 @@ def main() -> int:
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--network-mcs", type=int, default=2000)
     ap.add_argument("--network-shape", default="", help="BLOCKxSLOTS of the network (default: autotuned)")
+    ap.add_argument("--network-shape-after-tune", default="",
+                    help="BLOCKxSLOTS the networks use after the autotuner has run (default: the tuner's pick)")
     ap.add_argument("--batch-shape-after-tune", default="",
                     help="BLOCKxSLOTS the batch uses after the autotuner has run (default: the tuner's pick)")
     ap.add_argument("--no-network", action="store_true")
@@ -664,12 +672,15 @@ def main() -> int:
     u.cleanup()
     if not args.no_network and args.network_mcs > 0:
         nshape = tuple(int(v) for v in args.network_shape.split("x")) if args.network_shape else None
+        nafter = (tuple(int(v) for v in args.network_shape_after_tune.split("x"))
+                  if args.network_shape_after_tune else None)
         if args.network in ("both", "dense"):
             out["network"] = network_section(args.seed, args.network_mcs, kind, c_l, nshape, args.clear_pass,
-                                             SLOT_ORDER[args.slot_order], dense=True)
+                                             SLOT_ORDER[args.slot_order], dense=True, after_tune=nafter)
         if args.network in ("both", "grown"):
             out["network_grown"] = network_section(args.seed, args.network_mcs, kind, c_l, nshape,
-                                                   args.clear_pass, SLOT_ORDER[args.slot_order], dense=False)
+                                                   args.clear_pass, SLOT_ORDER[args.slot_order], dense=False,
+                                                   after_tune=nafter)
     if not args.no_batch and args.replicas > 0:
         out["batched"] = batched_vs_plain(args.replicas, args.replica_mcs, args.seed, kind, args.tune_sample,
                                           args.tune_mcs, plain=not args.no_plain,
