@@ -225,6 +225,40 @@ int main(int argc, char** argv) {
             for (size_t k = 0; k + 1 < pairs.size(); k += 2) s.bonds.push_back({pairs[k], pairs[k + 1]});
             check_oracle(s, check_layout(s, "star network, 128x32x64", false), "star network, 128x32x64");
         }
+        // dense ordered stars, moved by 20 sweeps of the oracle, then two
+        // rounds of crosslinks made at the moved corners
+        {
+            Sys s{{64, 64, 64}, {}, {}};
+            std::vector<uint32_t> pairs;
+            expect(place_stars_ordered(s.box, 120, 8, 13, s.xyz, pairs), "ordered stars");
+            for (size_t k = 0; k + 1 < pairs.size(); k += 2) s.bonds.push_back({pairs[k], pairs[k + 1]});
+            const std::vector<uint8_t> colors = check_layout(s, "ordered stars", false);
+            const Adjacency a = adjacency(s);
+            uint8_t forbidden[512];
+            oracle_standard_bondset(forbidden);
+            uint32_t nColors = 0;
+            for (uint8_t c : colors) nColors = std::max<uint32_t>(nColors, c + 1u);
+            expect(nColors == 2, "ordered stars: two species");
+            oracle_system sys{};
+            std::memcpy(sys.box, s.box, sizeof sys.box);
+            sys.n = s.n();
+            sys.xyz = s.xyz.data();
+            sys.nbr_start = a.start.data();
+            sys.nbr = a.ids.data();
+            sys.color = colors.data();
+            sys.n_colors = nColors;
+            sys.forbidden = forbidden;
+            sys.seed = 13;
+            expect(oracle_check_system(&sys) == 0, "ordered stars: valid start state");
+            std::vector<uint32_t> log(20 * nColors);
+            expect(oracle_run(&sys, 0, log.size(), log.data(), 1, 0) >= 0, "ordered stars: oracle run");
+            HostRng rng(13);
+            std::vector<uint32_t> cross;
+            crosslink_arm_ends(s.box, s.xyz, 120, 4, 8, 2, rng, cross);
+            expect(!cross.empty(), "ordered stars: crosslinks formed after the run");
+            for (size_t k = 0; k + 1 < cross.size(); k += 2) s.bonds.push_back({cross[k], cross[k + 1]});
+            check_oracle(s, check_layout(s, "ordered stars, crosslinked", false), "ordered stars, crosslinked");
+        }
         // the reference's fixtures: read, lay out, write and read back
         for (const char* f : {"LinearChain_l1024_n64.bfm.gz", "Ring_N6_n6.bfm", "Hydrogel_NStar_117_NoPerXYZ64.bfm"}) {
             const BfmData b = read_bfm(golden + "/" + f);
