@@ -236,10 +236,22 @@ def network_section(seed: int, mcs: int, kind: int, c_l: float, shape=None, clea
     _, glog = v.run(3, log=True)
     gx = v.getPositions(n)
     v.cleanup()
+    import oracle
+
+    nc = int(colors.max()) + 1
+    threads = oracle.max_threads()
     o = _oracle_of(sd, colors)
-    _, olog = o.run(0, 3 * (int(colors.max()) + 1))
+    tic = time.perf_counter()
+    _, olog = o.run(0, 3 * nc, threads=threads)
     parity = bool(np.array_equal(gx, o.xyz) and np.array_equal(glog, olog))
     log(f"network parity over 3 MCS: {parity}")
+    # the oracle continues from there: the CPU baseline of this workload
+    cpu_mcs = 30
+    o.run(3 * nc, (cpu_mcs - 3) * nc, threads=threads)
+    cpu_s = time.perf_counter() - tic
+    sizes = np.bincount(colors, minlength=nc)
+    cpu_attempted = sum(int(sizes[P.substep_draw(seed, k, nc)[0]]) for k in range(cpu_mcs * nc))
+    log(f"network CPU oracle: {cpu_attempted / cpu_s / 1e6:.1f} M attempts/s on {threads} threads")
     fraction = 8 * n / (box[0] * box[1] * box[2])
     if dense:
         what = (f"dense star network: 12800 4-arm stars (arm 29) = {n} monomers on {box[0]}^3 periodic, "
@@ -261,6 +273,10 @@ def network_section(seed: int, mcs: int, kind: int, c_l: float, shape=None, clea
         "shape": {"block": block, "slots": slots},
         "parity_vs_oracle": f"{'bit-exact' if parity else 'MISMATCH'} over 3 MCS",
     }
+    out["cpu_baseline"] = {"value": cpu_attempted / cpu_s, "unit": "attempted monomer moves/s", "cores": threads,
+                           "kind": "port",
+                           "sample": f"the first {cpu_mcs} MCS ({cpu_attempted} attempts) of the network, colored "
+                                     f"two-pass oracle (oracle/oracle.c) with OpenMP, {cpu_s:.1f} s"}
     if relax_parity is not None:
         out["build_run_vs_oracle"] = (f"{'bit-exact' if relax_parity else 'MISMATCH'} over "
                                       f"{W.DENSE_NETWORK['relax_mcs']} MCS of the stars")
