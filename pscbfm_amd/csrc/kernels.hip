@@ -229,7 +229,12 @@ __device__ __forceinline__ void clear_other_parity(const SysDesc& d, uint64_t t,
 // Pass 1 of substep t for logical workgroup lb of nb of one system.  The
 // survivors go to `list` (capacity SLOTS * blockDim), their number to *count.
 // With `clear` it also zeroes slice lb of nb of the other parity's proposals.
-template <int LAT, int SLOTS, int MAXDEG>
+// KEYED: the system's slots are in spatial order and each slot's draw key is
+// read from its `key` piece; else the key is the hash of the slot index.  A
+// compile-time choice: as a runtime branch between the load and the hash
+// inside the load batch, the compiler's wait for the loaded key also lands on
+// the hashing path, which then waits for every load issued before it.
+template <int LAT, int SLOTS, int MAXDEG, bool KEYED>
 __device__ __forceinline__ void check_body(const SysDesc& d, uint64_t t, SubstepDraw draw, uint32_t lb, uint32_t nb,
                                            bool clear, gptr<uint64_t> list, gptr<uint32_t> count,
                                            uint32_t* sForbidden, uint32_t* sCount) {
@@ -250,7 +255,6 @@ __device__ __forceinline__ void check_body(const SysDesc& d, uint64_t t, Substep
     // order, the key of its draw: read from the slot's key when the slots
     // are in spatial order, else the hash of the slot index itself.
     const gptr<const uint32_t> keys = d.key.global() + ci.slotOffset;
-    const bool keyed = d.keyed != 0u;
     // Partner ids: the bit-lattice kernel reads the full matrix rows; the
     // uint8-lattice kernel, whose plane words cost more traffic, reads the
     // 16-bit codes (half the bytes) and the full matrix only for escapes.
@@ -275,7 +279,7 @@ __device__ __forceinline__ void check_body(const SysDesc& d, uint64_t t, Substep
         live[s] = j[s] < ci.count;
         const uint32_t jj = live[s] ? j[s] : 0u;  // count > 0 for every species
         m[s] = pos[jj];
-        if (keyed)
+        if constexpr (KEYED)
             hj[s] = keys[jj];
         else
             hj[s] = move_hash(jj);
@@ -435,14 +439,14 @@ __device__ __forceinline__ void perform_body(const SysDesc& d, const Launch& L, 
 }
 
 // ---- single system ----------------------------------------------------------------
-template <int LAT, int SLOTS, int MAXDEG>
+template <int LAT, int SLOTS, int MAXDEG, bool KEYED>
 __global__ __launch_bounds__(256) void k_check(SysDesc d, uint32_t k, uint32_t clear) {
     __shared__ uint32_t sForbidden[16];
     __shared__ uint32_t sCount;
     Launch L;
     if (!begin_substep(d, k, L)) return;
     const uint32_t lb = xcd_block();
-    check_body<LAT, SLOTS, MAXDEG>(d, L.t, L.draw, lb, gridDim.x, clear != 0u,
+    check_body<LAT, SLOTS, MAXDEG, KEYED>(d, L.t, L.draw, lb, gridDim.x, clear != 0u,
                                    (gptr<uint64_t>)d.list.global() + size_t(lb) * (SLOTS * blockDim.x),
                                    d.listCount.global() + lb, sForbidden, &sCount);
 }
@@ -543,9 +547,14 @@ __global__ __launch_bounds__(256) void k_check_batch(const SysDesc* __restrict__
     const SysDesc& d = descs[tile.replica];
     Launch L;
     if (!begin_substep(d, k, L)) return;
-    check_body<LAT, SLOTS, MAXDEG>(d, L.t, L.draw, tile.chunk, tile.chunks, false,
-                                   global_ptr(reinterpret_cast<uint64_t*>(lists)) + size_t(blockIdx.x) * (SLOTS * blockDim.x),
-                                   global_ptr(listCounts) + blockIdx.x, sForbidden, &sCount);
+    const gptr<uint64_t> list = global_ptr(reinterpret_cast<uint64_t*>(lists)) + size_t(blockIdx.x) * (SLOTS * blockDim.x);
+    // replicas of either slot order share a launch: one uniform branch per workgroup
+    if (d.keyed != 0u)
+        check_body<LAT, SLOTS, MAXDEG, true>(d, L.t, L.draw, tile.chunk, tile.chunks, false, list,
+                                             global_ptr(listCounts) + blockIdx.x, sForbidden, &sCount);
+    else
+        check_body<LAT, SLOTS, MAXDEG, false>(d, L.t, L.draw, tile.chunk, tile.chunks, false, list,
+                                              global_ptr(listCounts) + blockIdx.x, sForbidden, &sCount);
 }
 
 template <int LAT>
@@ -683,8 +692,11 @@ void check_launch(const SysDesc& d, const BatchLaunch* b, uint32_t k, const Laun
     if (b)
         hipLaunchKernelGGL((k_check_batch<LAT, SLOTS, MAXDEG>), dim3(b->tiles), blk, 0, st, b->descs, b->tileTable,
                            b->lists, b->listCounts, k);
+    else if (d.keyed != 0u)
+        hipLaunchKernelGGL((k_check<LAT, SLOTS, MAXDEG, true>), dim3(s.grid), blk, 0, st, d, k,
+                           uint32_t(s.clearPass == 1u));
     else
-        hipLaunchKernelGGL((k_check<LAT, SLOTS, MAXDEG>), dim3(s.grid), blk, 0, st, d, k,
+        hipLaunchKernelGGL((k_check<LAT, SLOTS, MAXDEG, false>), dim3(s.grid), blk, 0, st, d, k,
                            uint32_t(s.clearPass == 1u));
 }
 
