@@ -335,18 +335,26 @@ __device__ __forceinline__ void check_body(const SysDesc& d, uint64_t t, Substep
         ok[s] = live[s] && !(bad & 1u) && occ[s] == 0u;
     }
 
-    // Mark targets and append survivors to this workgroup's list.
+    // Mark targets and append survivors to this workgroup's list: one LDS
+    // counter update per wave for all its slots.
     const gptr<uint32_t> T = proposal + size_t(t & 1u) * d.latWords;
     const uint32_t lane = threadIdx.x & 63u;
+    uint64_t mask[SLOTS];
+    uint32_t before[SLOTS], total = 0;
 #pragma unroll
     for (int s = 0; s < SLOTS; ++s) {
-        const uint64_t mask = __ballot(ok[s]);
-        uint32_t waveBase = 0;
-        if (lane == 0 && mask != 0) waveBase = atomicAdd(sCount, uint32_t(__popcll(mask)));
-        waveBase = __shfl(waveBase, 0, 64);
+        mask[s] = __ballot(ok[s]);
+        before[s] = total;
+        total += uint32_t(__popcll(mask[s]));
+    }
+    uint32_t waveBase = 0;
+    if (lane == 0 && total != 0) waveBase = atomicAdd(sCount, total);
+    waveBase = __shfl(waveBase, 0, 64);
+#pragma unroll
+    for (int s = 0; s < SLOTS; ++s) {
         if (ok[s]) {
             __hip_atomic_fetch_or(&T[to[s] >> 5], 1u << (to[s] & 31u), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            list[waveBase + lane_rank(mask)] = pack_entry(j[s] | (dir[s] << kDirShift), m[s]);
+            list[waveBase + before[s] + lane_rank(mask[s])] = pack_entry(j[s] | (dir[s] << kDirShift), m[s]);
         }
     }
     __syncthreads();
