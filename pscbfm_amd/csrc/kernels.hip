@@ -242,9 +242,12 @@ __device__ __forceinline__ void check_body(const SysDesc& d, uint64_t t, Substep
     const ColorInfo ci = load_color(d.colors.global(), draw.species);
     const gptr<uint32_t> proposal = d.proposal.global();
 
-    if (threadIdx.x < 16) sForbidden[threadIdx.x] = d.bondMask.global()[threadIdx.x];
-    if (threadIdx.x == 0) *sCount = 0;
-    __syncthreads();
+    // The bond table (16 words, the same for every lane: scalar loads) is
+    // read now and goes to LDS after the loads below are issued: the barrier
+    // that publishes it waits only in front of the bond test.
+    uint32_t forbidden[16];
+#pragma unroll
+    for (int w = 0; w < 16; ++w) forbidden[w] = d.bondMask.global()[w];
 
     const uint32_t B = blockDim.x;
     const uint32_t base = lb * (SLOTS * B);
@@ -323,6 +326,12 @@ __device__ __forceinline__ void check_body(const SysDesc& d, uint64_t t, Substep
             partnerCode[s][r] = posAll[partner[s][r] != kNoNeighbor ? partner[s][r] : 0u];
         occ[s] = plane_any<LAT == kLatticeU8 ? 3 : 6>(lattice, mv);
     }
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int w = 0; w < 16; ++w) sForbidden[w] = forbidden[w];
+        *sCount = 0;
+    }
+    __syncthreads();
     bool ok[SLOTS];
 #pragma unroll
     for (int s = 0; s < SLOTS; ++s) {
