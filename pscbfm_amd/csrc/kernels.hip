@@ -370,16 +370,27 @@ __device__ __forceinline__ void check_body(const SysDesc& d, uint64_t t, Substep
     if (threadIdx.x == 0) *count = *sCount;
 }
 
-__device__ __forceinline__ uint32_t block_sum(uint32_t v, uint32_t* scratch) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
-    if (lane == 0) scratch[wave] = v;
-    __syncthreads();
+// Sum over the wave of small per-lane counts, bit by bit with ballots and
+// scalar popcounts (a uniform loop over the bits any lane has set).
+__device__ __forceinline__ uint32_t wave_sum_small(uint32_t v) {
     uint32_t total = 0;
-    if (threadIdx.x == 0)
-        for (uint32_t w = 0; w < (blockDim.x + 63u) >> 6; ++w) total += scratch[w];
+    for (uint32_t b = 0; __ballot((v >> b) != 0u) != 0u; ++b)
+        total += uint32_t(__popcll(__ballot(((v >> b) & 1u) != 0u))) << b;
     return total;
+}
+
+// Adds a wave's accepted moves to its workgroup's running total and to the
+// substep's log entry, without waiting: atomics whose results are unused.
+__device__ __forceinline__ void add_accepted(const Launch& L, gptr<uint64_t> accSlot, uint32_t accepted) {
+    const uint32_t total = wave_sum_small(accepted);
+    if ((threadIdx.x & 63u) == 0u && total != 0u) {
+        __hip_atomic_fetch_add(&accSlot[0], uint64_t(total), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (L.log != nullptr) {
+            const uint64_t idx = L.t - L.logFirst;
+            if (idx < L.logCapacity)
+                __hip_atomic_fetch_add(&L.log[idx], total, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+    }
 }
 
 // Applies an accepted move of the monomer in `slot` from code m in
@@ -412,12 +423,11 @@ __device__ __forceinline__ void apply_move(const SysDesc& d, gptr<uint32_t> pos,
 }
 
 // Pass 2 of substep t over one pass-1 list.  The accepted count is added to
-// *accSlot, an entry only this workgroup writes.  With `clear` it also zeroes
-// slice lb of nb of the other parity's proposals.
+// *accSlot, an entry only this workgroup's waves add to.  With `clear` it
+// also zeroes slice lb of nb of the other parity's proposals.
 template <int LAT>
 __device__ __forceinline__ void perform_body(const SysDesc& d, const Launch& L, uint32_t lb, uint32_t nb, bool clear,
-                                             gptr<const uint64_t> list, uint32_t count, gptr<uint64_t> accSlot,
-                                             uint32_t* sSum) {
+                                             gptr<const uint64_t> list, uint32_t count, gptr<uint64_t> accSlot) {
     const uint64_t t = L.t;
     const SubstepDraw draw = L.draw;
     const ColorInfo ci = load_color(d.colors.global(), draw.species);
@@ -444,15 +454,7 @@ __device__ __forceinline__ void perform_body(const SysDesc& d, const Launch& L, 
         apply_move<LAT>(d, pos, lat, ci.slotOffset + jj, m, dir, mv);
         ++accepted;
     }
-    const uint32_t total = block_sum(accepted, sSum);
-    if (threadIdx.x == 0 && total != 0u) {
-        *accSlot += total;
-        if (L.log != nullptr) {
-            const uint64_t idx = t - L.logFirst;
-            if (idx < L.logCapacity)
-                __hip_atomic_fetch_add(&L.log[idx], total, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        }
-    }
+    add_accepted(L, accSlot, accepted);
 }
 
 // ---- single system ----------------------------------------------------------------
@@ -470,13 +472,12 @@ __global__ __launch_bounds__(256) void k_check(SysDesc d, uint32_t k, uint32_t c
 
 template <int LAT>
 __global__ __launch_bounds__(256) void k_perform(SysDesc d, uint32_t k, uint32_t listStride, uint32_t clear) {
-    __shared__ uint32_t sSum[4];
     Launch L;
     if (!begin_substep(d, k, L)) return;
     const uint32_t lb = xcd_block();  // same numbering as pass 1: list lb
     perform_body<LAT>(d, L, lb, gridDim.x, clear != 0u,
                       (gptr<const uint64_t>)d.list.global() + size_t(lb) * listStride, d.listCount.global()[lb],
-                      L.acceptedByBlock + blockIdx.x, sSum);
+                      L.acceptedByBlock + blockIdx.x);
 }
 
 // Pass 2 of a single system with several lists per workgroup: each group
@@ -489,7 +490,6 @@ __global__ __launch_bounds__(256) void k_perform(SysDesc d, uint32_t k, uint32_t
 template <int LAT, int DEPTH>
 __global__ __launch_bounds__(256) void k_perform_lists(SysDesc d, uint32_t k, uint32_t listStride,
                                                        uint32_t listsPerXcd, uint32_t group, uint32_t clear) {
-    __shared__ uint32_t sSum[4];
     const uint32_t xcd = blockIdx.x & 7u, local = blockIdx.x >> 3;
     const uint32_t gi = threadIdx.x / group, lane = threadIdx.x - gi * group;
     const uint32_t li = local * (blockDim.x / group) + gi;
@@ -540,15 +540,7 @@ __global__ __launch_bounds__(256) void k_perform_lists(SysDesc d, uint32_t k, ui
             ++accepted;
         }
     }
-    const uint32_t total = block_sum(accepted, sSum);
-    if (threadIdx.x == 0 && total != 0u) {
-        L.acceptedByBlock[blockIdx.x] += total;
-        if (L.log != nullptr) {
-            const uint64_t idx = t - L.logFirst;
-            if (idx < L.logCapacity)
-                __hip_atomic_fetch_add(&L.log[idx], total, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        }
-    }
+    add_accepted(L, L.acceptedByBlock + blockIdx.x, accepted);
 }
 
 // ---- batch of replicas ---------------------------------------------------------------
@@ -578,7 +570,6 @@ template <int LAT>
 __global__ __launch_bounds__(256) void k_perform_batch(const SysDesc* __restrict__ descs,
                                                        const Tile* __restrict__ tiles, const uint2* lists,
                                                        const uint32_t* listCounts, uint32_t k, uint32_t listStride) {
-    __shared__ uint32_t sSum[4];
     const Tile tile = tiles[blockIdx.x];
     if (tile.replica == kNoReplica) return;
     const SysDesc& d = descs[tile.replica];
@@ -586,7 +577,7 @@ __global__ __launch_bounds__(256) void k_perform_batch(const SysDesc* __restrict
     if (!begin_substep(d, k, L)) return;
     perform_body<LAT>(d, L, tile.chunk, tile.chunks, true,
                       global_ptr(reinterpret_cast<const uint64_t*>(lists)) + size_t(blockIdx.x) * listStride,
-                      global_ptr(listCounts)[blockIdx.x], L.acceptedByBlock + tile.chunk, sSum);
+                      global_ptr(listCounts)[blockIdx.x], L.acceptedByBlock + tile.chunk);
 }
 
 __global__ void k_advance(DevState* st, uint32_t count) {
