@@ -374,7 +374,7 @@ __device__ __forceinline__ void check_body(const SysDesc& d, uint64_t t, Substep
 // scalar popcounts (a uniform loop over the bits any lane has set).
 __device__ __forceinline__ uint32_t wave_sum_small(uint32_t v) {
     uint32_t total = 0;
-    for (uint32_t b = 0; __ballot((v >> b) != 0u) != 0u; ++b)
+    for (uint32_t b = 0; b < 32u && __ballot((v >> b) != 0u) != 0u; ++b)
         total += uint32_t(__popcll(__ballot(((v >> b) & 1u) != 0u))) << b;
     return total;
 }
