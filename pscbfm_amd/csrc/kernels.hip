@@ -242,9 +242,10 @@ __device__ __forceinline__ void check_body(const SysDesc& d, uint64_t t, Substep
     const ColorInfo ci = load_color(d.colors.global(), draw.species);
     const gptr<uint32_t> proposal = d.proposal.global();
 
-    // The bond table (16 words, the same for every lane: scalar loads) is
-    // read now and goes to LDS after the loads below are issued: the barrier
-    // that publishes it waits only in front of the bond test.
+    // The bond table (16 words, the same for every lane: scalar loads) goes
+    // to LDS after the gathers below are issued, and the barrier that
+    // publishes it stands in front of the bond test.  A barrier ahead of the
+    // gathers held every wave's loads back by the table's round trip.
     uint32_t forbidden[16];
 #pragma unroll
     for (int w = 0; w < 16; ++w) forbidden[w] = d.bondMask.global()[w];
